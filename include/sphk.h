@@ -1,0 +1,174 @@
+/*
+ * sphk.h -- C ABI of the MI355X SPH step library (libsphk.so).
+ *
+ * The reference has no FFI: its boundary is the C++ SPHEngine class
+ * (src/sph_engine.h:91-140) and the pybind11 module built from
+ * python/bindings.cpp.  Each entry point below replaces one piece of that
+ * class; the C++ SPHEngine and the `sph` Python module of this project are
+ * thin layers over these calls.
+ *
+ * Conventions
+ *  - Every function returns int status: 0 = ok, negative = error (SPHK_E_*).
+ *    The message of the last error on the calling thread is available from
+ *    sphk_last_error().  No C++ exception crosses this boundary.
+ *  - Host pointers are plain arrays; xyz quantities are interleaved (n*3 floats).
+ *  - One engine owns one HIP stream on one device.  Calls on one engine must
+ *    not overlap from several threads.  sphk_step() enqueues work and returns;
+ *    every read-back call synchronises the engine's stream first.
+ *  - Particle order on input and output is the caller's order (ids 0..n-1),
+ *    as in ParticleSystem (src/particle.cpp:26-33: id = index).
+ */
+#ifndef SPHK_H
+#define SPHK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPHK_OK 0
+#define SPHK_E_INVALID (-1)   /* bad argument */
+#define SPHK_E_HIP (-2)       /* HIP runtime error */
+#define SPHK_E_CAPACITY (-3)  /* particle count above capacity */
+#define SPHK_E_DOMAIN (-4)    /* positions need a cell grid beyond the supported size */
+#define SPHK_E_NODEVICE (-5)  /* no GPU visible */
+
+typedef struct sphk_engine sphk_engine;
+
+/* Same fields and defaults as SPHParameters (src/sph_engine.h:14-34), flattened.
+ * `formulation` is an addition: 0 (default) = the reference's formulas,
+ * 1 = consistent SPH (kernel normalised by 3/(2 pi h^3), acceleration = F / rho_i). */
+typedef struct {
+    float rest_density;
+    float gas_constant;
+    float viscosity;
+    float smoothing_length;
+    float particle_mass;
+    float timestep;
+    float gravity;
+    float damping;
+    float CFL_factor;
+    float xmin, xmax, ymin, ymax, zmin, zmax;
+    float neighbor_search_radius;
+    int32_t formulation;
+} sphk_params;
+
+/* Phase timings and neighbour statistics of the steps since the last reset,
+ * mirroring SPHEngine::PerformanceStats (src/sph_engine.h:51-59).  Times in seconds,
+ * measured with HIP events recorded on the engine's stream around each phase when
+ * timing is enabled (no synchronisation per step; events are read by sphk_get_stats).
+ * neighbor_search_time covers cell keys, sort, cell table and gather;
+ * force_computation_time covers the fused force + leapfrog + clamp kernel, so
+ * integration_time stays 0. */
+typedef struct {
+    double total_time;
+    double neighbor_search_time;
+    double density_computation_time;
+    double force_computation_time;
+    double integration_time;
+    uint64_t max_neighbors;
+    uint64_t total_neighbor_queries;
+    uint64_t total_neighbors;  /* sum of neighbour-list lengths (incl. self) */
+    uint64_t steps;
+    uint64_t timed_steps;      /* steps whose phases were timed (timing enabled) */
+} sphk_stats;
+
+/* read-back selectors for sphk_download */
+#define SPHK_POSITIONS 0     /* n*3 */
+#define SPHK_VELOCITIES 1    /* n*3 */
+#define SPHK_DENSITIES 2     /* capacity entries (SPHEngine::get_densities, src/sph_engine.h:135) */
+#define SPHK_PRESSURES 3     /* capacity entries (src/sph_engine.h:136) */
+#define SPHK_ACCELERATIONS 4 /* n*3, the accelerations_ buffer (src/sph_engine.h:64) */
+#define SPHK_MASSES 5        /* n */
+
+/* Default parameter values of SPHParameters{} (src/sph_engine.h:14-34). */
+int sphk_default_params(sphk_params* out);
+
+/* Lifecycle.  Replaces SPHEngine(size_t max_particles) (src/sph_engine.cpp:13-18). */
+int sphk_create(size_t capacity, sphk_engine** out);
+int sphk_destroy(sphk_engine* e);
+
+/* Replaces SPHEngine::initialize / set_parameters / set_boundaries
+ * (src/sph_engine.cpp:20-33,152-172): stores the parameters used by later steps. */
+int sphk_set_params(sphk_engine* e, const sphk_params* p);
+int sphk_get_params(const sphk_engine* e, sphk_params* out);
+
+/* Writes particles [offset, offset+n) and sets the particle count to offset+n.
+ * offset must not exceed the current count.  vel may be NULL (zero velocity);
+ * mass may be NULL (params.particle_mass).  Replaces ParticleSystem::add_particles
+ * (src/particle.cpp:35-39) when offset == current count, and clear+add when 0.
+ * Particles beyond capacity are dropped and SPHK_E_CAPACITY is returned after the
+ * ones that fit are stored, as ParticleSystem::add_particle warns and drops
+ * (src/particle.cpp:26-33). */
+int sphk_upload(sphk_engine* e, size_t offset, size_t n, const float* pos_xyz,
+                const float* vel_xyz, const float* mass);
+/* Particle count (ParticleSystem::size). */
+int sphk_size(const sphk_engine* e, size_t* out);
+/* Resets the particle count, simulated time and step counter
+ * (SPHEngine::clear_particles, src/sph_engine.cpp:87-91). */
+int sphk_clear(sphk_engine* e);
+
+/* One SPH step (SPHEngine::step, src/sph_engine.cpp:93-144): cell grid build,
+ * density, pressure, forces, leapfrog, wall clamp.  dt <= 0 selects the CFL
+ * timestep (src/sph_engine.cpp:312-333), computed on the device.  Enqueued on
+ * the engine's stream.  A no-op when the engine holds no particles. */
+int sphk_step(sphk_engine* e, float dt);
+/* Runs num_steps steps (SPHEngine::run_steps, src/sph_engine.cpp:146-150). */
+int sphk_run_steps(sphk_engine* e, size_t num_steps, int adaptive_timestep);
+
+/* Read-backs (synchronise the stream). */
+int sphk_download(sphk_engine* e, int which, float* out, size_t count);
+int sphk_get_time(sphk_engine* e, float* out);
+int sphk_get_step_count(const sphk_engine* e, uint64_t* out);
+/* get_total_mass (sum of densities * h^3), get_total_energy (kinetic),
+ * compute_conservation_errors (src/sph_engine.cpp:178-200). */
+int sphk_total_mass(sphk_engine* e, float* out);
+int sphk_total_energy(sphk_engine* e, float* out);
+int sphk_conservation_errors(sphk_engine* e, float* mass_error, float* energy_error);
+
+/* Performance stats (SPHEngine::get_performance_stats / reset_performance_stats). */
+int sphk_set_timing(sphk_engine* e, int enabled);
+int sphk_get_stats(sphk_engine* e, sphk_stats* out);
+int sphk_reset_stats(sphk_engine* e);
+
+/* Device-resident snapshot of the particle state (positions, velocities,
+ * accelerations, time) and its restore, both device-to-device on the stream.
+ * No reference counterpart: used for reset-and-step measurement and for
+ * surrogate training trajectories. */
+int sphk_snapshot_save(sphk_engine* e);
+int sphk_snapshot_restore(sphk_engine* e);
+
+/* Synchronises the engine's stream. */
+int sphk_synchronize(sphk_engine* e);
+
+/* ---- parity/debug read-backs (run the cell-grid build for the current state) ----
+ * sphk_debug_cell_coords: per particle (caller order) the integer cell coordinates
+ *   floor(p * (1/cell)) of SpatialHash::get_grid_coords (src/spatial_hash.h:30-36),
+ *   n*3 int32; non-finite coordinates give INT32_MIN like x86's conversion.
+ * sphk_debug_cell_keys: the 63-bit SpatialHash::hash_position keys
+ *   (src/spatial_hash.h:20-27), n uint64.
+ * sphk_debug_sorted: the cell-sorted order (caller ids, ascending within a cell,
+ *   as SpatialHash::build appends them, src/spatial_hash.cpp:19-24) and the
+ *   per-particle dense cell index; n entries each.
+ * sphk_debug_neighbors: neighbour lists in SpatialHash::query_squared order
+ *   (src/spatial_hash.cpp:31-57), CSR over caller ids; call with idx == NULL to
+ *   get the total in *total, then again with buffers of that size. */
+int sphk_debug_cell_coords(sphk_engine* e, int32_t* coords_xyz, size_t n);
+int sphk_debug_cell_keys(sphk_engine* e, uint64_t* keys, size_t n);
+int sphk_debug_sorted(sphk_engine* e, uint32_t* order, uint32_t* cell_of_sorted, size_t n);
+int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t cap,
+                         size_t* total);
+/* Grid shape of the last build: origin cell (3), dims (3), number of cells. */
+int sphk_debug_grid(sphk_engine* e, int32_t* origin3, int32_t* dims3, uint64_t* ncells);
+
+/* Last error message of the calling thread; returns its length. */
+int sphk_last_error(char* buf, size_t len);
+/* Library version string ("1.0.0", the reference module's __version__). */
+const char* sphk_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,114 @@
+/*
+ * sph_oracle.h -- CPU restatement of the reference SPH step (TEST INFRASTRUCTURE).
+ *
+ * This library is the checker for the HIP path, not part of the product: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * It restates, operation for operation in IEEE FP32 (no contraction, no
+ * reassociation), the reference hot path:
+ *   SpatialHash::get_grid_coords/hash_position  src/spatial_hash.h:20-36
+ *   SpatialHash::build                          src/spatial_hash.cpp:15-25
+ *   SpatialHash::query_squared                  src/spatial_hash.cpp:31-57, src/spatial_hash.h:70-73
+ *   CubicSplineKernel W/gradW/laplacianW (3-D)  src/kernels.cpp:23-36,58-66,96-108,130-153
+ *   equations::compute_density .. acceleration  src/sph_engine.cpp:370-443
+ *   SPHEngine::step                             src/sph_engine.cpp:93-144
+ *   SPHEngine::integrate_leapfrog               src/sph_engine.cpp:290-310
+ *   ParticleSystem::apply_boundary_conditions   src/particle.cpp:122-153
+ *   SPHEngine::compute_cfl_timestep             src/sph_engine.cpp:312-333
+ *   get_total_mass / get_total_energy           src/sph_engine.cpp:187-200
+ * plus an FP64 evaluation of density and acceleration over the same neighbour
+ * lists (the error yardstick for the FP32 paths).
+ *
+ * Pinning status of each function is stated in oracle/README.md and is checked by
+ * tests/test_oracle.py (closed forms, brute force, lattice invariants).
+ */
+#ifndef SPH_ORACLE_H
+#define SPH_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Field order and defaults follow SPHParameters (src/sph_engine.h:14-34). */
+typedef struct {
+    float rest_density;
+    float gas_constant;
+    float viscosity;
+    float smoothing_length;
+    float particle_mass;
+    float timestep;
+    float gravity;
+    float damping;
+    float CFL_factor;
+    float xmin, xmax, ymin, ymax, zmin, zmax;
+    float neighbor_search_radius;
+    int32_t formulation; /* 0 = reference formulas, 1 = consistent (sigma=3/(2 pi h^3), a=F/rho_i) */
+} or_params;
+
+typedef struct or_state or_state;
+
+/* which-selectors for or_get */
+enum { OR_POS = 0, OR_VEL = 1, OR_DENSITY = 2, OR_PRESSURE = 3, OR_ACCEL = 4, OR_MASS = 5 };
+
+void      or_default_params(or_params* p);
+or_state* or_create(size_t capacity);
+void      or_destroy(or_state* s);
+void      or_set_params(or_state* s, const or_params* p);
+/* Replaces the particle set (ids = index).  pos/vel are xyz-interleaved, n*3 floats. */
+int       or_set_particles(or_state* s, size_t n, const float* pos, const float* vel, const float* mass);
+size_t    or_size(const or_state* s);
+/* Full reference step; dt <= 0 selects the CFL timestep.  Returns the dt used. */
+float     or_step(or_state* s, float dt);
+/* Sub-phases, in step order (for phase-by-phase parity). */
+void      or_update_neighbor_lists(or_state* s);
+void      or_compute_densities(or_state* s);
+void      or_compute_pressures(or_state* s);
+void      or_compute_forces(or_state* s);
+void      or_integrate_leapfrog(or_state* s, float dt);
+void      or_apply_boundary_conditions(or_state* s);
+float     or_compute_cfl_timestep(const or_state* s);
+/* Copies N rows (3 floats per row for POS/VEL/ACCEL, 1 otherwise). */
+void      or_get(const or_state* s, int which, float* out);
+/* Neighbour lists in reference order (CSR). offsets has n+1 entries. */
+size_t    or_neighbor_total(const or_state* s);
+void      or_get_neighbors(const or_state* s, int64_t* offsets, int64_t* idx);
+size_t    or_max_neighbors(const or_state* s);
+float     or_total_mass(const or_state* s);
+float     or_total_energy(const or_state* s);
+/* Number of hash buckets (occupied cells) after the last build. */
+size_t    or_num_cells(const or_state* s);
+
+/* Standalone pieces (pinned by closed forms). */
+uint64_t  or_hash_position(int32_t x, int32_t y, int32_t z);
+void      or_grid_coords(float inv_cell, const float* p, int32_t* c);
+void      or_cell_keys(size_t n, const float* pos, float cell_size, int32_t* coords, uint64_t* keys);
+float     or_W(const float* r, float h, int32_t formulation);
+void      or_gradW(const float* r, float h, int32_t formulation, float* out);
+float     or_laplacianW(const float* r, float h, int32_t formulation);
+/* Leapfrog + wall clamp on caller arrays (n particles), in place. */
+void      or_integrate_clamp(size_t n, float* pos, float* vel, const float* acc, float dt,
+                             const or_params* p);
+
+/* FP64 density and acceleration over given neighbour lists (reference order).
+ * pos/vel/mass are FP32 inputs widened to double. */
+void      or64_density_accel(size_t n, const float* pos, const float* vel, const float* mass,
+                             const int64_t* offsets, const int64_t* idx, const or_params* p,
+                             double* rho, double* acc);
+/* Per-particle sum of |pressure+viscosity force terms| / m (the conditioning yardstick for
+ * acceleration comparisons), FP64 over the given lists. */
+void      or64_accel_term_scale(size_t n, const float* pos, const float* vel, const float* mass,
+                                const float* rho, const int64_t* offsets, const int64_t* idx,
+                                const or_params* p, double* scale);
+/* Brute-force O(n^2) neighbour sets with the same d^2 operation order (ascending j). */
+size_t    or_brute_neighbors(size_t n, const float* pos, float r2, int64_t* offsets, int64_t* idx,
+                             size_t cap);
+
+int       or_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,976 @@
+// sphk_api.hip -- C ABI of libsphk: engine lifecycle, cell-grid sizing, step orchestration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sphk_internal.h"
+
+using namespace sphk;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return set_error(SPHK_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e));   \
+    } while (0)
+
+// Largest dense grid (cells incl. padding) an engine will build.  4 bytes per cell.
+constexpr uint64_t kMaxCells = 1ull << 28;
+// Cell coordinates beyond +-2^30 cannot be represented in the dense grid.
+constexpr double kMaxCellCoord = 1073741824.0;
+
+template <class T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+}  // namespace
+
+struct sphk_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t cap = 0;
+    size_t n = 0;
+    sphk_params params{};
+    uint64_t step_count = 0;
+
+    // home arrays (caller order)
+    float4* pm = nullptr;
+    float4* vel = nullptr;
+    float4* acc = nullptr;
+    float* rho = nullptr;
+    float* prs = nullptr;
+    // snapshot
+    float4* snap_pm = nullptr;
+    float4* snap_vel = nullptr;
+    float4* snap_acc = nullptr;
+    size_t snap_n = 0;
+    uint64_t snap_steps = 0;
+    float snap_ext_lo[3] = {0, 0, 0}, snap_ext_hi[3] = {0, 0, 0};
+    bool snap_ext_any = false, snap_ext_unknown = false;
+    bool has_snapshot = false;
+    // work arrays
+    uint32_t* keys = nullptr;
+    uint32_t* keys_alt = nullptr;
+    uint32_t* vals = nullptr;
+    uint32_t* vals_alt = nullptr;
+    uint32_t* hist = nullptr;
+    float4* spm = nullptr;
+    float4* svr = nullptr;
+    uint32_t* cell_start = nullptr;
+    uint64_t cell_start_cap = 0;
+    uint32_t* skey = nullptr;  // -> keys or keys_alt after the sort
+    uint32_t* sid = nullptr;   // -> vals or vals_alt after the sort
+    // device scalars: [0] dt, [1] time, [2] snapshot time
+    float* fscal = nullptr;
+    // [0] out-of-grid flag, [1] cfl scratch
+    uint32_t* uscal = nullptr;
+    // [0] max neighbours, [1] total neighbours
+    unsigned long long* nstats = nullptr;
+
+    // grid
+    Grid grid;
+    bool grid_valid = false;
+    float ext_lo[3] = {0, 0, 0}, ext_hi[3] = {0, 0, 0};
+    bool ext_any = false;      // ext holds at least one finite position
+    bool ext_unknown = false;  // positions may lie outside ext (needs a device bbox)
+    StepConsts consts{};
+    bool sorted_valid = false;  // work arrays hold the cell-sorted image of the current state
+
+    // timing: one event set per timed step, drained when stats are read
+    struct EventSet {
+        hipEvent_t ev[4];
+    };
+    bool timing = false;
+    std::vector<EventSet> pending, free_sets;
+    sphk_stats stats{};
+};
+
+namespace {
+
+void fill_default(sphk_params* p) {
+    p->rest_density = 1000.0f;
+    p->gas_constant = 2000.0f;
+    p->viscosity = 0.001f;
+    p->smoothing_length = 0.02f;
+    p->particle_mass = 0.001f;
+    p->timestep = 0.001f;
+    p->gravity = -9.81f;
+    p->damping = 0.99f;
+    p->CFL_factor = 0.4f;
+    p->xmin = -1.0f;
+    p->xmax = 1.0f;
+    p->ymin = -1.0f;
+    p->ymax = 1.0f;
+    p->zmin = -1.0f;
+    p->zmax = 1.0f;
+    p->neighbor_search_radius = 0.04f;
+    p->formulation = 0;
+}
+
+// Host-side constants, in the reference's FP32 expression order.
+StepConsts make_consts(const sphk_params& p, const Grid& g) {
+#pragma clang fp contract(off)
+    StepConsts c{};
+    const float pi_f = 3.14159265358979323846f;
+    const float h = p.smoothing_length;
+    c.inv_cell = 1.0f / p.neighbor_search_radius;  // SpatialHash::set_cell_size (src/spatial_hash.h:63-66)
+    c.r2 = p.neighbor_search_radius * p.neighbor_search_radius;  // src/sph_engine.cpp:347
+    c.h = h;
+    c.inv_h = 1.0f / h;
+    if (p.formulation == 1)
+        c.sigma = 3.0f / (2.0f * pi_f * h * h * h);
+    else
+        c.sigma = 1.0f / (pi_f * h * h * h);  // src/kernels.cpp:27
+    float q0 = 0.0f;
+    float w0 = (2.0f / 3.0f - q0 * q0) + ((0.5f * q0) * q0) * q0;
+    c.w0 = c.sigma * w0;
+    c.grad_scale = c.sigma / h;
+    float h_sq = h * h;
+    c.lap_scale = (c.sigma * p.viscosity) / h_sq;
+    c.k_gas = p.gas_constant;
+    c.rho0 = p.rest_density;
+    c.grav = p.gravity;
+    c.damping = p.damping;
+    c.lo[0] = p.xmin;
+    c.hi[0] = p.xmax;
+    c.lo[1] = p.ymin;
+    c.hi[1] = p.ymax;
+    c.lo[2] = p.zmin;
+    c.hi[2] = p.zmax;
+    c.formulation = p.formulation;
+    c.ox = g.ox;
+    c.oy = g.oy;
+    c.oz = g.oz;
+    c.nx = g.nx;
+    c.ny = g.ny;
+    c.nz = g.nz;
+    c.G = (uint32_t)g.G;
+    return c;
+}
+
+int floor_cell(float x, float inv, int64_t* out) {
+    float s = x * inv;
+    double f = std::floor((double)s);
+    if (!(f >= -kMaxCellCoord && f < kMaxCellCoord)) return -1;
+    *out = (int64_t)f;
+    return 0;
+}
+
+void ext_add_box(sphk_engine* e, const float lo[3], const float hi[3]) {
+    for (int d = 0; d < 3; ++d) {
+        if (!e->ext_any) {
+            e->ext_lo[d] = lo[d];
+            e->ext_hi[d] = hi[d];
+        } else {
+            e->ext_lo[d] = std::min(e->ext_lo[d], lo[d]);
+            e->ext_hi[d] = std::max(e->ext_hi[d], hi[d]);
+        }
+    }
+    e->ext_any = true;
+}
+
+int bounds_box(const sphk_params& p, float lo[3], float hi[3]) {
+    const float a[3] = {p.xmin, p.ymin, p.zmin};
+    const float b[3] = {p.xmax, p.ymax, p.zmax};
+    for (int d = 0; d < 3; ++d) {
+        if (!std::isfinite(a[d]) || !std::isfinite(b[d])) return -1;
+        lo[d] = std::min(a[d], b[d]);
+        hi[d] = std::max(a[d], b[d]);
+    }
+    return 0;
+}
+
+// Dense grid covering every cell a finite position in [ext_lo, ext_hi] can fall in,
+// padded by two cells per side so the +-2 walk never leaves the grid.
+int grid_for_ext(const sphk_engine* e, Grid* g) {
+    const float inv = 1.0f / e->params.neighbor_search_radius;
+    int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (e->ext_any) {
+        const float* l = e->ext_lo;
+        const float* h = e->ext_hi;
+        for (int d = 0; d < 3; ++d) {
+            if (floor_cell(l[d], inv, &lo[d]) || floor_cell(h[d], inv, &hi[d]))
+                return set_error(SPHK_E_DOMAIN,
+                                 "particle coordinates need cell indices beyond +-2^30");
+        }
+    }
+    int64_t dims[3];
+    uint64_t G = 1;
+    for (int d = 0; d < 3; ++d) {
+        dims[d] = hi[d] - lo[d] + 1 + 4;
+        G *= (uint64_t)dims[d];
+        if (G > kMaxCells)
+            return set_error(SPHK_E_DOMAIN,
+                             "positions span more than %llu cells of size %g; the dense cell "
+                             "grid supports at most that many",
+                             (unsigned long long)kMaxCells, (double)e->params.neighbor_search_radius);
+    }
+    g->ox = (int32_t)(lo[0] - 2);
+    g->oy = (int32_t)(lo[1] - 2);
+    g->oz = (int32_t)(lo[2] - 2);
+    g->nx = (int32_t)dims[0];
+    g->ny = (int32_t)dims[1];
+    g->nz = (int32_t)dims[2];
+    g->G = G;
+    return SPHK_OK;
+}
+
+__global__ void k_bbox(const float4* __restrict__ pm, uint32_t n, float* __restrict__ out6,
+                       uint32_t* __restrict__ any) {
+    // out6 = {min x,y,z as ordered-uint, max x,y,z} over finite positions
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool found = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 p = pm[i];
+        if (__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z)) {
+            lo[0] = fminf(lo[0], p.x);
+            lo[1] = fminf(lo[1], p.y);
+            lo[2] = fminf(lo[2], p.z);
+            hi[0] = fmaxf(hi[0], p.x);
+            hi[1] = fmaxf(hi[1], p.y);
+            hi[2] = fmaxf(hi[2], p.z);
+            found = true;
+        }
+    }
+    unsigned* u = reinterpret_cast<unsigned*>(out6);
+    auto ord = [](float f) {
+        unsigned b = __float_as_uint(f);
+        return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    };
+    if (found) {
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&u[d], ord(lo[d]));
+            atomicMax(&u[3 + d], ord(hi[d]));
+        }
+        atomicOr(any, 1u);
+    }
+}
+
+float unord(unsigned b) {
+    unsigned r = (b & 0x80000000u) ? (b & 0x7fffffffu) : ~b;
+    float f;
+    memcpy(&f, &r, 4);
+    return f;
+}
+
+// Device bounding box of the finite positions (synchronous; used only when the
+// wall bounds are too wide to bound the grid).
+int device_bbox(sphk_engine* e) {
+    uint32_t* buf = nullptr;
+    HIP_TRY(hipMalloc(&buf, 8 * sizeof(uint32_t)));
+    uint32_t init[8] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u, 0u, 0u};
+    HIP_TRY(hipMemcpyAsync(buf, init, sizeof(init), hipMemcpyHostToDevice, e->stream));
+    if (e->n) {
+        uint32_t blocks = (uint32_t)std::min<size_t>((e->n + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_bbox, dim3(blocks), dim3(256), 0, e->stream, e->pm, (uint32_t)e->n,
+                           reinterpret_cast<float*>(buf), buf + 6);
+        HIP_TRY(hipGetLastError());
+    }
+    uint32_t out[8];
+    HIP_TRY(hipMemcpyAsync(out, buf, sizeof(out), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipFree(buf));
+    e->ext_any = out[6] != 0;
+    for (int d = 0; d < 3; ++d) {
+        e->ext_lo[d] = e->ext_any ? unord(out[d]) : 0.0f;
+        e->ext_hi[d] = e->ext_any ? unord(out[3 + d]) : 0.0f;
+    }
+    e->ext_unknown = false;
+    return SPHK_OK;
+}
+
+int ensure_grid(sphk_engine* e) {
+    if (e->ext_unknown) {
+        int rc = device_bbox(e);
+        if (rc) return rc;
+    }
+    Grid g;
+    int rc = grid_for_ext(e, &g);
+    if (rc) return rc;
+    if (g.G + 2 > e->cell_start_cap) {
+        if (e->cell_start) HIP_TRY(hipFree(e->cell_start));
+        uint64_t want = std::max<uint64_t>(g.G + 2, e->cell_start_cap * 5 / 4);
+        want = std::min<uint64_t>(want, kMaxCells + 2);
+        HIP_TRY(dalloc(&e->cell_start, (size_t)want));
+        e->cell_start_cap = want;
+    }
+    e->grid = g;
+    e->grid_valid = true;
+    e->consts = make_consts(e->params, g);
+    return SPHK_OK;
+}
+
+int total_key_bits(uint64_t G) {
+    int b = 1;
+    while ((1ull << b) < G + 1) ++b;
+    return b;
+}
+
+// keys -> sort -> cell_start -> gather.  Leaves e->skey/e->sid/e->spm/e->svr valid.
+int build_sorted(sphk_engine* e) {
+    const uint32_t n = (uint32_t)e->n;
+    hipStream_t st = e->stream;
+    HIP_TRY(launch_cell_keys(e->pm, n, e->consts, e->keys, e->uscal, st));
+    uint32_t* ko = nullptr;
+    uint32_t* vo = nullptr;
+    HIP_TRY(radix_sort_keys(e->keys, e->keys_alt, e->vals, e->vals_alt, e->hist, n,
+                            total_key_bits(e->grid.G), &ko, &vo, st));
+    e->skey = ko;
+    e->sid = vo;
+    HIP_TRY(launch_cell_start(e->skey, n, (uint32_t)e->grid.G, e->cell_start, st));
+    HIP_TRY(launch_gather(e->pm, e->vel, e->sid, n, e->spm, e->svr, st));
+    e->sorted_valid = true;
+    return SPHK_OK;
+}
+
+// A finite particle outside the dense grid is routed to the non-finite bin and
+// flagged; the grid sizing above makes this unreachable, so report it loudly.
+int check_grid_flag(sphk_engine* e) {
+    uint32_t flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, e->uscal, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (flag) return set_error(SPHK_E_DOMAIN, "a finite particle fell outside the cell grid");
+    return SPHK_OK;
+}
+
+// Folds the events of completed timed steps into the phase totals.
+int drain_timing(sphk_engine* e) {
+    for (auto& set : e->pending) {
+        HIP_TRY(hipEventSynchronize(set.ev[3]));
+        float t01, t12, t23, t03;
+        HIP_TRY(hipEventElapsedTime(&t01, set.ev[0], set.ev[1]));
+        HIP_TRY(hipEventElapsedTime(&t12, set.ev[1], set.ev[2]));
+        HIP_TRY(hipEventElapsedTime(&t23, set.ev[2], set.ev[3]));
+        HIP_TRY(hipEventElapsedTime(&t03, set.ev[0], set.ev[3]));
+        e->stats.neighbor_search_time += t01 * 1e-3;
+        e->stats.density_computation_time += t12 * 1e-3;
+        // the force kernel also integrates and clamps (one fused pass)
+        e->stats.force_computation_time += t23 * 1e-3;
+        e->stats.total_time += t03 * 1e-3;
+        e->stats.timed_steps++;
+        e->free_sets.push_back(set);
+    }
+    e->pending.clear();
+    return SPHK_OK;
+}
+
+int check_engine(const sphk_engine* e) {
+    if (!e) return set_error(SPHK_E_INVALID, "null engine");
+    return SPHK_OK;
+}
+
+int set_device(const sphk_engine* e) {
+    HIP_TRY(hipSetDevice(e->device));
+    return SPHK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sphk_default_params(sphk_params* out) {
+    if (!out) return set_error(SPHK_E_INVALID, "null params");
+    fill_default(out);
+    return SPHK_OK;
+}
+
+int sphk_create(size_t capacity, sphk_engine** out) {
+    if (!out) return set_error(SPHK_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (capacity == 0) return set_error(SPHK_E_INVALID, "capacity must be positive");
+    if (capacity > 0x7fffffffull) return set_error(SPHK_E_INVALID, "capacity above 2^31-1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_error(SPHK_E_NODEVICE, "no HIP device visible");
+    sphk_engine* e = new sphk_engine();
+    e->cap = capacity;
+    fill_default(&e->params);
+    HIP_TRY(hipGetDevice(&e->device));
+    HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    const size_t C = capacity;
+    HIP_TRY(dalloc(&e->pm, C));
+    HIP_TRY(dalloc(&e->vel, C));
+    HIP_TRY(dalloc(&e->acc, C));
+    HIP_TRY(dalloc(&e->rho, C));
+    HIP_TRY(dalloc(&e->prs, C));
+    HIP_TRY(dalloc(&e->snap_pm, C));
+    HIP_TRY(dalloc(&e->snap_vel, C));
+    HIP_TRY(dalloc(&e->snap_acc, C));
+    HIP_TRY(dalloc(&e->keys, C));
+    HIP_TRY(dalloc(&e->keys_alt, C));
+    HIP_TRY(dalloc(&e->vals, C));
+    HIP_TRY(dalloc(&e->vals_alt, C));
+    HIP_TRY(dalloc(&e->hist, ((size_t)1 << kRadixMaxBits) * ((C + kRadixTile - 1) / kRadixTile)));
+    HIP_TRY(dalloc(&e->spm, C));
+    HIP_TRY(dalloc(&e->svr, C));
+    HIP_TRY(dalloc(&e->fscal, 4));
+    HIP_TRY(dalloc(&e->uscal, 4));
+    HIP_TRY(dalloc(&e->nstats, 2));
+    HIP_TRY(hipMemsetAsync(e->acc, 0, C * sizeof(float4), e->stream));
+    HIP_TRY(hipMemsetAsync(e->vel, 0, C * sizeof(float4), e->stream));
+    HIP_TRY(hipMemsetAsync(e->rho, 0, C * sizeof(float), e->stream));
+    HIP_TRY(hipMemsetAsync(e->prs, 0, C * sizeof(float), e->stream));
+    HIP_TRY(hipMemsetAsync(e->fscal, 0, 4 * sizeof(float), e->stream));
+    HIP_TRY(hipMemsetAsync(e->uscal, 0, 4 * sizeof(uint32_t), e->stream));
+    HIP_TRY(hipMemsetAsync(e->nstats, 0, 2 * sizeof(unsigned long long), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    *out = e;
+    return SPHK_OK;
+}
+
+int sphk_destroy(sphk_engine* e) {
+    if (!e) return SPHK_OK;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    void* ptrs[] = {e->pm,   e->vel,      e->acc,  e->rho,      e->prs,  e->snap_pm,
+                    e->snap_vel, e->snap_acc, e->keys, e->keys_alt, e->vals, e->vals_alt,
+                    e->hist, e->spm,      e->svr,  e->cell_start, e->fscal, e->uscal,
+                    e->nstats};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto* v : {&e->pending, &e->free_sets})
+        for (auto& set : *v)
+            for (auto& ev : set.ev) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return SPHK_OK;
+}
+
+int sphk_set_params(sphk_engine* e, const sphk_params* p) {
+    if (int rc = check_engine(e)) return rc;
+    if (!p) return set_error(SPHK_E_INVALID, "null params");
+    if (!(p->neighbor_search_radius > 0.0f) || !std::isfinite(p->neighbor_search_radius))
+        return set_error(SPHK_E_INVALID, "neighbor_search_radius must be positive and finite");
+    if (!(p->smoothing_length > 0.0f) || !std::isfinite(p->smoothing_length))
+        return set_error(SPHK_E_INVALID, "smoothing_length must be positive and finite");
+    if (p->formulation != 0 && p->formulation != 1)
+        return set_error(SPHK_E_INVALID, "formulation must be 0 or 1");
+    e->params = *p;
+    e->grid_valid = false;
+    e->sorted_valid = false;
+    return SPHK_OK;
+}
+
+int sphk_get_params(const sphk_engine* e, sphk_params* out) {
+    if (int rc = check_engine(e)) return rc;
+    if (!out) return set_error(SPHK_E_INVALID, "null params");
+    *out = e->params;
+    return SPHK_OK;
+}
+
+int sphk_upload(sphk_engine* e, size_t offset, size_t n, const float* pos, const float* vel,
+                const float* mass) {
+    if (int rc = check_engine(e)) return rc;
+    if (offset > e->n) return set_error(SPHK_E_INVALID, "offset %zu beyond count %zu", offset, e->n);
+    if (n && !pos) return set_error(SPHK_E_INVALID, "null positions");
+    if (int rc = set_device(e)) return rc;
+    size_t fit = std::min(n, e->cap - offset);
+    std::vector<float4> hp(fit), hv(fit);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool any = false;
+    const float inv = 1.0f / e->params.neighbor_search_radius;
+    for (size_t i = 0; i < fit; ++i) {
+        float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        float m = mass ? mass[i] : e->params.particle_mass;
+        hp[i] = make_float4(x, y, z, m);
+        hv[i] = vel ? make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0f)
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (std::isfinite(x) && std::isfinite(y) && std::isfinite(z)) {
+            int64_t cc;
+            if (floor_cell(x, inv, &cc) || floor_cell(y, inv, &cc) || floor_cell(z, inv, &cc))
+                return set_error(SPHK_E_DOMAIN,
+                                 "particle %zu at (%g, %g, %g) needs a cell index beyond +-2^30",
+                                 offset + i, (double)x, (double)y, (double)z);
+            lo[0] = std::min(lo[0], x);
+            lo[1] = std::min(lo[1], y);
+            lo[2] = std::min(lo[2], z);
+            hi[0] = std::max(hi[0], x);
+            hi[1] = std::max(hi[1], y);
+            hi[2] = std::max(hi[2], z);
+            any = true;
+        }
+    }
+    if (fit) {
+        HIP_TRY(hipMemcpyAsync(e->pm + offset, hp.data(), fit * sizeof(float4),
+                               hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->vel + offset, hv.data(), fit * sizeof(float4),
+                               hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    if (offset == 0) {
+        e->ext_any = false;
+        e->ext_unknown = false;
+    }
+    if (any) ext_add_box(e, lo, hi);
+    e->n = offset + fit;
+    e->grid_valid = false;
+    e->sorted_valid = false;
+    if (fit < n)
+        return set_error(SPHK_E_CAPACITY, "particle capacity %zu exceeded: %zu particles dropped",
+                         e->cap, n - fit);
+    return SPHK_OK;
+}
+
+int sphk_size(const sphk_engine* e, size_t* out) {
+    if (int rc = check_engine(e)) return rc;
+    if (!out) return set_error(SPHK_E_INVALID, "null output");
+    *out = e->n;
+    return SPHK_OK;
+}
+
+int sphk_clear(sphk_engine* e) {
+    if (int rc = check_engine(e)) return rc;
+    if (int rc = set_device(e)) return rc;
+    e->n = 0;
+    e->step_count = 0;
+    e->ext_any = false;
+    e->ext_unknown = false;
+    e->grid_valid = false;
+    e->sorted_valid = false;
+    HIP_TRY(hipMemsetAsync(e->fscal + 1, 0, sizeof(float), e->stream));
+    return SPHK_OK;
+}
+
+int sphk_step(sphk_engine* e, float dt) {
+    if (int rc = check_engine(e)) return rc;
+    if (e->n == 0) return SPHK_OK;  // SPHEngine::step returns when empty (src/sph_engine.cpp:94)
+    if (int rc = set_device(e)) return rc;
+    if (!e->grid_valid || e->ext_unknown) {
+        if (int rc = ensure_grid(e)) return rc;
+    }
+    hipStream_t st = e->stream;
+    const uint32_t n = (uint32_t)e->n;
+    const bool adaptive = !(dt > 0.0f);
+    hipEvent_t* ev = nullptr;
+    if (e->timing) {
+        if (e->free_sets.empty()) {
+            sphk_engine::EventSet set;
+            for (auto& x : set.ev) HIP_TRY(hipEventCreate(&x));
+            e->free_sets.push_back(set);
+        }
+        e->pending.push_back(e->free_sets.back());
+        e->free_sets.pop_back();
+        ev = e->pending.back().ev;
+        HIP_TRY(hipEventRecord(ev[0], st));
+    }
+    if (adaptive) HIP_TRY(launch_cfl(e->vel, e->acc, n, e->params, e->uscal + 1, e->fscal, st));
+    if (int rc = build_sorted(e)) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[1], st));
+    HIP_TRY(launch_density(e->spm, e->svr, e->skey, e->cell_start, n, e->consts,
+                           reinterpret_cast<uint32_t*>(e->nstats), st));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], st));
+    HIP_TRY(launch_force_integrate(e->spm, e->svr, e->skey, e->sid, e->cell_start, n, e->consts,
+                                   adaptive ? e->fscal : nullptr, dt, e->pm, e->vel, e->acc,
+                                   e->rho, e->prs, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[3], st));
+    HIP_TRY(launch_advance_time(e->fscal + 1, adaptive ? e->fscal : nullptr, dt, st));
+    e->sorted_valid = false;
+    e->step_count++;
+    e->stats.total_neighbor_queries += n;
+    e->stats.steps++;
+    // after the wall clamp every finite position lies in the bounds box
+    float lo[3], hi[3];
+    if (bounds_box(e->params, lo, hi) == 0) {
+        Grid g;
+        e->ext_any = true;
+        for (int d = 0; d < 3; ++d) {
+            e->ext_lo[d] = lo[d];
+            e->ext_hi[d] = hi[d];
+        }
+        std::string keep = g_last_error;
+        int rc = grid_for_ext(e, &g);
+        if (rc == SPHK_OK) {
+            if (g.G != e->grid.G || g.ox != e->grid.ox || g.oy != e->grid.oy || g.oz != e->grid.oz ||
+                g.nx != e->grid.nx || g.ny != e->grid.ny || g.nz != e->grid.nz)
+                e->grid_valid = false;
+        } else {
+            e->ext_unknown = true;  // walls too wide to bound the grid: measure the live bbox next step
+            g_last_error = keep;
+        }
+    } else {
+        e->ext_unknown = true;
+    }
+    return SPHK_OK;
+}
+
+int sphk_run_steps(sphk_engine* e, size_t num_steps, int adaptive_timestep) {
+    if (int rc = check_engine(e)) return rc;
+    for (size_t i = 0; i < num_steps; ++i) {
+        int rc = sphk_step(e, adaptive_timestep ? 0.0f : e->params.timestep);
+        if (rc) return rc;
+    }
+    return SPHK_OK;
+}
+
+int sphk_synchronize(sphk_engine* e) {
+    if (int rc = check_engine(e)) return rc;
+    if (int rc = set_device(e)) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return SPHK_OK;
+}
+
+int sphk_download(sphk_engine* e, int which, float* out, size_t count) {
+    if (int rc = check_engine(e)) return rc;
+    if (!out && count) return set_error(SPHK_E_INVALID, "null output");
+    if (int rc = set_device(e)) return rc;
+    if (int rc = check_grid_flag(e)) return rc;
+    const size_t n = e->n;
+    switch (which) {
+        case SPHK_POSITIONS:
+        case SPHK_VELOCITIES:
+        case SPHK_ACCELERATIONS: {
+            if (count < n * 3) return set_error(SPHK_E_INVALID, "need %zu floats", n * 3);
+            const float4* src = which == SPHK_POSITIONS ? e->pm
+                                : which == SPHK_VELOCITIES ? e->vel
+                                                           : e->acc;
+            std::vector<float4> tmp(n);
+            if (n) HIP_TRY(hipMemcpyAsync(tmp.data(), src, n * sizeof(float4), hipMemcpyDeviceToHost,
+                                          e->stream));
+            HIP_TRY(hipStreamSynchronize(e->stream));
+            for (size_t i = 0; i < n; ++i) {
+                out[3 * i] = tmp[i].x;
+                out[3 * i + 1] = tmp[i].y;
+                out[3 * i + 2] = tmp[i].z;
+            }
+            return SPHK_OK;
+        }
+        case SPHK_MASSES: {
+            if (count < n) return set_error(SPHK_E_INVALID, "need %zu floats", n);
+            std::vector<float4> tmp(n);
+            if (n) HIP_TRY(hipMemcpyAsync(tmp.data(), e->pm, n * sizeof(float4), hipMemcpyDeviceToHost,
+                                          e->stream));
+            HIP_TRY(hipStreamSynchronize(e->stream));
+            for (size_t i = 0; i < n; ++i) out[i] = tmp[i].w;
+            return SPHK_OK;
+        }
+        case SPHK_DENSITIES:
+        case SPHK_PRESSURES: {
+            size_t m = std::min(count, e->cap);
+            HIP_TRY(hipMemcpyAsync(out, which == SPHK_DENSITIES ? e->rho : e->prs, m * sizeof(float),
+                                   hipMemcpyDeviceToHost, e->stream));
+            HIP_TRY(hipStreamSynchronize(e->stream));
+            return SPHK_OK;
+        }
+        default:
+            return set_error(SPHK_E_INVALID, "unknown download selector %d", which);
+    }
+}
+
+int sphk_get_time(sphk_engine* e, float* out) {
+    if (int rc = check_engine(e)) return rc;
+    if (!out) return set_error(SPHK_E_INVALID, "null output");
+    if (int rc = set_device(e)) return rc;
+    HIP_TRY(hipMemcpyAsync(out, e->fscal + 1, sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return SPHK_OK;
+}
+
+int sphk_get_step_count(const sphk_engine* e, uint64_t* out) {
+    if (int rc = check_engine(e)) return rc;
+    if (!out) return set_error(SPHK_E_INVALID, "null output");
+    *out = e->step_count;
+    return SPHK_OK;
+}
+
+int sphk_total_mass(sphk_engine* e, float* out) {
+    // get_total_mass: std::accumulate(densities_[0..N)) * h^3 (src/sph_engine.cpp:187-190)
+    if (int rc = check_engine(e)) return rc;
+    if (!out) return set_error(SPHK_E_INVALID, "null output");
+    std::vector<float> d(e->n);
+    if (e->n) {
+        int rc = sphk_download(e, SPHK_DENSITIES, d.data(), e->n);
+        if (rc) return rc;
+    }
+    float acc = 0.0f;
+    for (float v : d) acc = acc + v;
+    float h = e->params.smoothing_length;
+    *out = acc * ((h * h) * h);
+    return SPHK_OK;
+}
+
+int sphk_total_energy(sphk_engine* e, float* out) {
+    // get_total_energy: sum 0.5 m |v|^2 (src/sph_engine.cpp:192-200)
+    if (int rc = check_engine(e)) return rc;
+    if (!out) return set_error(SPHK_E_INVALID, "null output");
+    if (int rc = set_device(e)) return rc;
+    const size_t n = e->n;
+    std::vector<float4> v(n), pm(n);
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(v.data(), e->vel, n * sizeof(float4), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(pm.data(), e->pm, n * sizeof(float4), hipMemcpyDeviceToHost, e->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    float ke = 0.0f;
+    for (size_t i = 0; i < n; ++i) {
+        float tx = v[i].x * v[i].x, ty = v[i].y * v[i].y, tz = v[i].z * v[i].z;
+        float sp = (tx + ty) + tz;
+        ke += (0.5f * pm[i].w) * sp;
+    }
+    *out = ke;
+    return SPHK_OK;
+}
+
+int sphk_conservation_errors(sphk_engine* e, float* mass_error, float* energy_error) {
+    // compute_conservation_errors (src/sph_engine.cpp:178-185); energy_error is 0 there too
+    if (int rc = check_engine(e)) return rc;
+    float tm = 0.0f;
+    if (int rc = sphk_total_mass(e, &tm)) return rc;
+    float initial = (float)e->n * e->params.particle_mass;
+    if (mass_error) *mass_error = std::fabs(tm - initial) / initial;
+    if (energy_error) *energy_error = 0.0f;
+    return SPHK_OK;
+}
+
+int sphk_set_timing(sphk_engine* e, int enabled) {
+    if (int rc = check_engine(e)) return rc;
+    e->timing = enabled != 0;
+    return SPHK_OK;
+}
+
+int sphk_get_stats(sphk_engine* e, sphk_stats* out) {
+    if (int rc = check_engine(e)) return rc;
+    if (!out) return set_error(SPHK_E_INVALID, "null output");
+    if (int rc = set_device(e)) return rc;
+    unsigned long long ns[2];
+    HIP_TRY(hipMemcpyAsync(ns, e->nstats, sizeof(ns), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (int rc = drain_timing(e)) return rc;
+    *out = e->stats;
+    out->max_neighbors = ns[0];
+    out->total_neighbors = ns[1];
+    return SPHK_OK;
+}
+
+int sphk_reset_stats(sphk_engine* e) {
+    if (int rc = check_engine(e)) return rc;
+    if (int rc = set_device(e)) return rc;
+    if (int rc = drain_timing(e)) return rc;
+    e->stats = sphk_stats{};
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(e->nstats), 0u, 4, e->stream));
+    return SPHK_OK;
+}
+
+int sphk_snapshot_save(sphk_engine* e) {
+    if (int rc = check_engine(e)) return rc;
+    if (int rc = set_device(e)) return rc;
+    const size_t n = e->n;
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(e->snap_pm, e->pm, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->snap_vel, e->vel, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->snap_acc, e->acc, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(e->fscal + 2, e->fscal + 1, sizeof(float), hipMemcpyDeviceToDevice, e->stream));
+    e->snap_n = n;
+    e->snap_steps = e->step_count;
+    for (int d = 0; d < 3; ++d) {
+        e->snap_ext_lo[d] = e->ext_lo[d];
+        e->snap_ext_hi[d] = e->ext_hi[d];
+    }
+    e->snap_ext_any = e->ext_any;
+    e->snap_ext_unknown = e->ext_unknown;
+    e->has_snapshot = true;
+    return SPHK_OK;
+}
+
+int sphk_snapshot_restore(sphk_engine* e) {
+    if (int rc = check_engine(e)) return rc;
+    if (!e->has_snapshot) return set_error(SPHK_E_INVALID, "no snapshot saved");
+    if (int rc = set_device(e)) return rc;
+    const size_t n = e->snap_n;
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(e->pm, e->snap_pm, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->vel, e->snap_vel, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->acc, e->snap_acc, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(e->fscal + 1, e->fscal + 2, sizeof(float), hipMemcpyDeviceToDevice, e->stream));
+    e->n = n;
+    e->step_count = e->snap_steps;
+    e->sorted_valid = false;
+    for (int d = 0; d < 3; ++d) {
+        e->ext_lo[d] = e->snap_ext_lo[d];
+        e->ext_hi[d] = e->snap_ext_hi[d];
+    }
+    e->ext_any = e->snap_ext_any;
+    e->ext_unknown = e->snap_ext_unknown;
+    e->grid_valid = false;
+    return SPHK_OK;
+}
+
+// ---- parity/debug ----
+
+static int prepare_sorted(sphk_engine* e);
+
+int sphk_debug_cell_coords(sphk_engine* e, int32_t* coords, size_t count) {
+    // Cell coordinates as the GPU binned them: dense index of each particle decoded back to
+    // absolute cells.  Particles in the non-finite bin get the x86 conversion of
+    // static_cast<int>(std::floor(p * inv)) per axis (INT_MIN where non-finite or out of range),
+    // as SpatialHash::get_grid_coords yields on x86 (src/spatial_hash.h:30-36).
+    if (int rc = check_engine(e)) return rc;
+    if (!coords || count < e->n) return set_error(SPHK_E_INVALID, "need %zu rows", e->n);
+    if (int rc = prepare_sorted(e)) return rc;
+    const size_t n = e->n;
+    std::vector<uint32_t> sid(n), skey(n);
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(sid.data(), e->sid, n * 4, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(skey.data(), e->skey, n * 4, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<float> pos(n * 3);
+    if (int rc = sphk_download(e, SPHK_POSITIONS, pos.data(), pos.size())) return rc;
+    const float inv = 1.0f / e->params.neighbor_search_radius;
+    const Grid& g = e->grid;
+    for (size_t s = 0; s < n; ++s) {
+        uint32_t id = sid[s];
+        uint32_t L = skey[s];
+        int32_t* c = coords + 3 * (size_t)id;
+        if (L < g.G) {
+            uint32_t rz = L % (uint32_t)g.nz;
+            uint32_t t = L / (uint32_t)g.nz;
+            uint32_t ry = t % (uint32_t)g.ny;
+            uint32_t rx = t / (uint32_t)g.ny;
+            c[0] = g.ox + (int32_t)rx;
+            c[1] = g.oy + (int32_t)ry;
+            c[2] = g.oz + (int32_t)rz;
+        } else {
+            for (int d = 0; d < 3; ++d) {
+                float f = std::floor(pos[3 * id + d] * inv);
+                c[d] = (f >= -2147483648.0f && f < 2147483648.0f) ? (int32_t)f : INT32_MIN;
+            }
+        }
+    }
+    return SPHK_OK;
+}
+
+int sphk_debug_cell_keys(sphk_engine* e, uint64_t* keys, size_t count) {
+    if (int rc = check_engine(e)) return rc;
+    if (!keys || count < e->n) return set_error(SPHK_E_INVALID, "need %zu keys", e->n);
+    std::vector<int32_t> c(e->n * 3);
+    if (int rc = sphk_debug_cell_coords(e, c.data(), e->n)) return rc;
+    for (size_t i = 0; i < e->n; ++i) {
+        uint64_t hsh = 0;
+        hsh |= ((uint64_t)((uint32_t)c[3 * i] & 0x1FFFFFu) << 42);
+        hsh |= ((uint64_t)((uint32_t)c[3 * i + 1] & 0x1FFFFFu) << 21);
+        hsh |= ((uint64_t)((uint32_t)c[3 * i + 2] & 0x1FFFFFu));
+        keys[i] = hsh;
+    }
+    return SPHK_OK;
+}
+
+static int prepare_sorted(sphk_engine* e) {
+    if (int rc = set_device(e)) return rc;
+    if (!e->grid_valid || e->ext_unknown) {
+        if (int rc = ensure_grid(e)) return rc;
+    }
+    if (!e->sorted_valid && e->n) {
+        if (int rc = build_sorted(e)) return rc;
+    }
+    return SPHK_OK;
+}
+
+int sphk_debug_sorted(sphk_engine* e, uint32_t* order, uint32_t* cell_of_sorted, size_t count) {
+    if (int rc = check_engine(e)) return rc;
+    if (count < e->n) return set_error(SPHK_E_INVALID, "need %zu entries", e->n);
+    if (int rc = prepare_sorted(e)) return rc;
+    if (e->n) {
+        if (order)
+            HIP_TRY(hipMemcpyAsync(order, e->sid, e->n * 4, hipMemcpyDeviceToHost, e->stream));
+        if (cell_of_sorted)
+            HIP_TRY(hipMemcpyAsync(cell_of_sorted, e->skey, e->n * 4, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return SPHK_OK;
+}
+
+int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t cap,
+                         size_t* total) {
+    if (int rc = check_engine(e)) return rc;
+    if (int rc = prepare_sorted(e)) return rc;
+    const size_t n = e->n;
+    uint32_t* dcnt = nullptr;
+    HIP_TRY(dalloc(&dcnt, n));
+    HIP_TRY(launch_nbr_count(e->spm, e->skey, e->sid, e->cell_start, (uint32_t)n, e->consts, dcnt,
+                             e->stream));
+    std::vector<uint32_t> cnt(n);
+    if (n) HIP_TRY(hipMemcpyAsync(cnt.data(), dcnt, n * 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipFree(dcnt));
+    std::vector<int64_t> offs(n + 1);
+    int64_t o = 0;
+    for (size_t i = 0; i < n; ++i) {
+        offs[i] = o;
+        o += cnt[i];
+    }
+    offs[n] = o;
+    if (total) *total = (size_t)o;
+    if (offsets) memcpy(offsets, offs.data(), (n + 1) * sizeof(int64_t));
+    if (!idx) return SPHK_OK;
+    if (cap < (size_t)o) return set_error(SPHK_E_INVALID, "need %lld index slots", (long long)o);
+    int64_t* doffs = nullptr;
+    int64_t* didx = nullptr;
+    HIP_TRY(dalloc(&doffs, n + 1));
+    HIP_TRY(dalloc(&didx, (size_t)o));
+    HIP_TRY(hipMemcpyAsync(doffs, offs.data(), (n + 1) * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(launch_nbr_write(e->spm, e->skey, e->sid, e->cell_start, (uint32_t)n, e->consts, doffs,
+                             didx, e->stream));
+    if (o) HIP_TRY(hipMemcpyAsync(idx, didx, (size_t)o * 8, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipFree(doffs));
+    HIP_TRY(hipFree(didx));
+    return SPHK_OK;
+}
+
+int sphk_debug_grid(sphk_engine* e, int32_t* origin3, int32_t* dims3, uint64_t* ncells) {
+    if (int rc = check_engine(e)) return rc;
+    if (int rc = prepare_sorted(e)) return rc;
+    if (int rc = check_grid_flag(e)) return rc;
+    if (origin3) {
+        origin3[0] = e->grid.ox;
+        origin3[1] = e->grid.oy;
+        origin3[2] = e->grid.oz;
+    }
+    if (dims3) {
+        dims3[0] = e->grid.nx;
+        dims3[1] = e->grid.ny;
+        dims3[2] = e->grid.nz;
+    }
+    if (ncells) *ncells = e->grid.G;
+    return SPHK_OK;
+}
+
+int sphk_last_error(char* buf, size_t len) {
+    if (buf && len) {
+        size_t m = std::min(len - 1, g_last_error.size());
+        memcpy(buf, g_last_error.data(), m);
+        buf[m] = 0;
+    }
+    return (int)g_last_error.size();
+}
+
+const char* sphk_version(void) { return "1.0.0"; }
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// sphk_internal.h -- device-side layout and launcher declarations of libsphk.
+//
+// HBM layout (one engine, capacity C particles):
+//   home arrays, caller (id) order, float4 rows:
+//     pm [C]  (x, y, z, mass)         vel [C] (vx, vy, vz, 0)
+//     acc[C]  (ax, ay, az, 0)         rho[C], prs[C]  (capacity-length, like densities_/pressures_)
+//   per-step work arrays, cell-sorted order:
+//     skey[C]  dense cell index        sid[C]  caller id
+//     spm [C]  (x, y, z, mass)         svr[C]  (vx, vy, vz, rho)
+//     cell_start[G+2]  prefix of cell occupancy, z-fastest dense grid, G = nx*ny*nz;
+//                      dense index G is the bin of particles with non-finite positions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sphk.h"
+
+namespace sphk {
+
+// Kernel-side constants of one step.  Derived on the host from sphk_params with
+// the reference's own FP32 expressions (src/kernels.cpp:23-36, src/spatial_hash.h:63-66).
+struct StepConsts {
+    float inv_cell;  // 1.0f / neighbor_search_radius
+    float r2;        // neighbor_search_radius^2
+    float h, inv_h;
+    float sigma;     // 1/(pi h^3) (x 3/2 for the consistent formulation)
+    float w0;        // W(0) = sigma * (2/3)
+    float grad_scale;  // sigma / h          (applied once per particle)
+    float lap_scale;   // sigma * mu / h^2   (applied once per particle)
+    float k_gas, rho0, grav, damping;
+    float lo[3], hi[3];  // wall clamp bounds
+    int32_t formulation;
+    int32_t ox, oy, oz;  // grid origin cell (absolute cell coordinates)
+    int32_t nx, ny, nz;  // grid dims (z fastest)
+    uint32_t G;          // nx*ny*nz; particles with dense index G have non-finite positions
+};
+
+struct Grid {
+    int32_t ox = 0, oy = 0, oz = 0;
+    int32_t nx = 0, ny = 0, nz = 0;
+    uint64_t G = 0;
+};
+
+// radix sort tile geometry
+constexpr int kRadixThreads = 256;
+constexpr int kRadixWaves = kRadixThreads / 64;
+constexpr int kRadixItems = 8;  // keys per thread
+constexpr int kRadixTile = kRadixThreads * kRadixItems;
+constexpr int kRadixMaxBits = 11;
+
+// launchers (sphk_kernels.hip); all enqueue on `st`
+hipError_t launch_cell_keys(const float4* pm, uint32_t n, const StepConsts& c, uint32_t* keys,
+                            uint32_t* err_flag, hipStream_t st);
+hipError_t radix_sort_keys(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
+                           uint32_t* hist, uint32_t n, int total_bits, uint32_t** keys_out,
+                           uint32_t** vals_out, hipStream_t st);
+size_t radix_hist_words(uint32_t n, int total_bits);
+hipError_t launch_cell_start(const uint32_t* skey, uint32_t n, uint32_t G, uint32_t* cell_start,
+                             hipStream_t st);
+hipError_t launch_gather(const float4* pm, const float4* vel, const uint32_t* sid, uint32_t n,
+                         float4* spm, float4* svr, hipStream_t st);
+hipError_t launch_density(const float4* spm, float4* svr, const uint32_t* skey,
+                          const uint32_t* cell_start, uint32_t n, const StepConsts& c,
+                          uint32_t* nbr_stats, hipStream_t st);
+hipError_t launch_force_integrate(const float4* spm, const float4* svr, const uint32_t* skey,
+                                  const uint32_t* sid, const uint32_t* cell_start, uint32_t n,
+                                  const StepConsts& c, const float* dt_dev, float dt,
+                                  float4* pm, float4* vel, float4* acc, float* rho, float* prs,
+                                  hipStream_t st);
+hipError_t launch_cfl(const float4* vel, const float4* acc, uint32_t n, const sphk_params& p,
+                      uint32_t* scratch_u32, float* dt_dev, hipStream_t st);
+hipError_t launch_advance_time(float* time_dev, const float* dt_dev, float dt, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+hipError_t launch_nbr_count(const float4* spm, const uint32_t* skey, const uint32_t* sid,
+                            const uint32_t* cell_start, uint32_t n, const StepConsts& c,
+                            uint32_t* count_by_id, hipStream_t st);
+hipError_t launch_nbr_write(const float4* spm, const uint32_t* skey, const uint32_t* sid,
+                            const uint32_t* cell_start, uint32_t n, const StepConsts& c,
+                            const int64_t* offsets_by_id, int64_t* idx, hipStream_t st);
+
+}  // namespace sphk

@@ -1,0 +1,322 @@
+"""Parity of the HIP step (through the C ABI of libsphk.so) with the CPU oracle.
+
+Bars:
+  * cell coordinates / 63-bit hash keys, per-cell membership and order, neighbour lists
+    (content and order): bit-exact.
+  * density: relative error <= 1e-5 against the FP32 oracle; the FP64 error is reported.
+  * acceleration: |a_gpu - a_oracle| / S_i <= 1e-5 per particle, where S_i is the sum of the
+    magnitudes of the pressure and viscosity terms of particle i divided by m_i (the sum
+    cancels by up to 9 digits, so plain relative error on a is ill-conditioned).
+  * leapfrog + wall clamp: bit-exact given the same accelerations.
+"""
+import numpy as np
+import pytest
+
+from conftest import R2
+
+pytestmark = pytest.mark.gpu
+
+RHO_TOL = 1e-5
+ACC_TOL = 1e-5
+
+
+def _engine(sphk, scene, formulation=0, capacity=None):
+    e = sphk.Engine(capacity or scene.n, scene.params(sphk, formulation))
+    e.upload(scene.pos, scene.vel)
+    return e
+
+
+def _oracle(oracle, scene, formulation=0):
+    o = oracle.Oracle(scene.n, scene.oracle_params(oracle, formulation))
+    o.set_particles(scene.pos, scene.vel)
+    return o
+
+
+def _oracle_phases(o):
+    o.lib.or_update_neighbor_lists(o.h)
+    o.lib.or_compute_densities(o.h)
+    o.lib.or_compute_pressures(o.h)
+    o.lib.or_compute_forces(o.h)
+
+
+def _check_neighbors(e, o):
+    offs_g, idx_g = e.neighbors()
+    offs_o, idx_o = o.neighbors()
+    np.testing.assert_array_equal(offs_g, offs_o)
+    np.testing.assert_array_equal(idx_g, idx_o)
+    return offs_o, idx_o
+
+
+def _check_density(e, o, scene, params, label=""):
+    n = scene.n
+    rho_g = e.densities()[:n]
+    rho_o = o.get(0 + 2)  # OR_DENSITY
+    rel = np.abs(rho_g.astype(np.float64) - rho_o) / np.abs(rho_o)
+    assert rel.max() <= RHO_TOL, f"{label} density rel err {rel.max():.3e}"
+    return rho_g, rho_o, rel
+
+
+def test_cell_keys_bitexact_config0(sphk, oracle, scenes):
+    s = scenes.config0()
+    e = _engine(sphk, s)
+    coords_g = e.cell_coords()
+    keys_g = e.cell_keys()
+    coords_o, keys_o = oracle.cell_keys(s.pos, np.float32(0.04))
+    np.testing.assert_array_equal(coords_g, coords_o)
+    np.testing.assert_array_equal(keys_g, keys_o)
+    # jitter puts some particles below 0: cell -1 must appear
+    assert coords_o.min() == -1
+
+
+def test_sorted_order_matches_hash_buckets(sphk, oracle, scenes):
+    # SpatialHash::build appends ids in ascending order per key: the GPU's cell-sorted order,
+    # grouped by cell, must list exactly the same ids in the same order.
+    s = scenes.config0(seed=3)
+    e = _engine(sphk, s)
+    order, cells = e.sorted_order()
+    _, keys_o = oracle.cell_keys(s.pos, np.float32(0.04))
+    assert np.all(np.diff(cells.astype(np.int64)) >= 0)
+    assert sorted(order.tolist()) == list(range(s.n))
+    buckets = {}
+    for i, k in enumerate(keys_o.tolist()):
+        buckets.setdefault(k, []).append(i)
+    bounds = np.flatnonzero(np.diff(cells.astype(np.int64))) + 1
+    groups = np.split(order, bounds)
+    assert len(groups) == len(buckets)
+    for g in groups:
+        k = int(keys_o[g[0]])
+        assert g.tolist() == buckets[k]
+
+
+def test_neighbors_density_accel_step0_config0(sphk, oracle, scenes):
+    s = scenes.config0()
+    e = _engine(sphk, s)
+    o = _oracle(oracle, s)
+    _oracle_phases(o)
+    offs, idx = _check_neighbors(e, o)
+    e.step(s.dt)
+    params = s.oracle_params(oracle)
+    rho_g, rho_o, rel = _check_density(e, o, s, params, "config0")
+    mass = np.full(s.n, s.mass, np.float32)
+    rho64, acc64 = oracle.density_accel_f64(s.pos, s.vel, mass, offs, idx, params)
+    rel64 = np.abs(rho_g - rho64) / rho64
+    assert rel64.max() < 1e-5
+    acc_g = e.accelerations()
+    acc_o = o.get(4)
+    scale = oracle.accel_term_scale(s.pos, s.vel, mass, rho_o, offs, idx, params)
+    err = np.linalg.norm(acc_g.astype(np.float64) - acc_o, axis=1) / scale
+    assert err.max() <= ACC_TOL, f"accel err {err.max():.3e}"
+    err64 = np.linalg.norm(acc_g - acc64, axis=1) / scale
+    assert err64.max() <= ACC_TOL
+    # pressures capacity-length, P = k (rho - rho0)
+    prs = e.pressures()
+    np.testing.assert_allclose(prs[: s.n], np.float32(2000.0) * (rho_g - np.float32(1000.0)),
+                               rtol=1e-6)
+
+
+def test_integrate_clamp_bitexact(sphk, oracle, scenes):
+    s = scenes.config0(seed=11)
+    e = _engine(sphk, s)
+    e.step(s.dt)
+    acc = e.accelerations()
+    x1, v1 = oracle.integrate_clamp(s.pos, s.vel, acc, s.dt, s.oracle_params(oracle))
+    np.testing.assert_array_equal(e.positions(), x1)
+    np.testing.assert_array_equal(e.velocities(), v1)
+
+
+def test_neighbor_edge_pair_two_cells_apart(sphk, oracle):
+    # (-1e-10, y, z) is in cell -1, (0.04f, y, z) in cell 1, and they are neighbours (d^2 == R^2).
+    pos = np.array([[-1e-10, 0.5, 0.5], [0.04, 0.5, 0.5], [0.02, 0.5, 0.5],
+                    [0.5, -1e-10, 0.3], [0.5, 0.04, 0.3], [0.3, 0.3, 0.04], [0.3, 0.3, -1e-10]],
+                   np.float32)
+    p = sphk.default_params(xmin=-1, xmax=1, ymin=-1, ymax=1, zmin=-1, zmax=1)
+    e = sphk.Engine(len(pos), p)
+    e.upload(pos)
+    op = oracle.default_params()
+    o = oracle.Oracle(len(pos), op)
+    o.set_particles(pos)
+    o.lib.or_update_neighbor_lists(o.h)
+    offs, idx = _check_neighbors(e, o)
+    assert 1 in idx[offs[0]:offs[1]].tolist()
+    assert 4 in idx[offs[3]:offs[4]].tolist()
+    assert 6 in idx[offs[5]:offs[6]].tolist()
+
+
+def test_random_points_against_brute_force(sphk, oracle):
+    rng = np.random.default_rng(12)
+    pos = rng.uniform(-0.2, 0.3, size=(4000, 3)).astype(np.float32)
+    e = sphk.Engine(len(pos), sphk.default_params())
+    e.upload(pos)
+    offs, idx = e.neighbors()
+    boffs, bidx = oracle.brute_neighbors(pos, R2)
+    for i in range(len(pos)):
+        assert sorted(idx[offs[i]:offs[i + 1]].tolist()) == bidx[boffs[i]:boffs[i + 1]].tolist()
+
+
+def test_nonfinite_particles(sphk, oracle):
+    pos = np.array([[0.1, 0.1, 0.1], [np.nan, 0.1, 0.1], [0.11, 0.1, 0.1], [0.1, np.inf, 0.1],
+                    [0.12, 0.1, 0.1]], np.float32)
+    e = sphk.Engine(8, sphk.default_params())
+    e.upload(pos)
+    o = oracle.Oracle(8, oracle.default_params())
+    o.set_particles(pos)
+    o.lib.or_update_neighbor_lists(o.h)
+    _check_neighbors(e, o)
+    o.lib.or_compute_densities(o.h)
+    o.lib.or_compute_pressures(o.h)
+    o.lib.or_compute_forces(o.h)
+    e.step(0.001)
+    rho = e.densities()[:5]
+    ro = o.get(2)
+    np.testing.assert_allclose(rho, ro, rtol=1e-6)
+    np.testing.assert_array_equal(e.accelerations()[1], np.array([0, -9.81, 0], np.float32))
+    coords_g = e.cell_coords()
+    coords_o, _ = oracle.cell_keys(e.positions(), np.float32(0.04))
+    np.testing.assert_array_equal(coords_g, coords_o)
+
+
+def test_consistent_formulation_multistep(sphk, oracle, scenes):
+    # the opt-in consistent formulation stays bounded at dt = 1e-4: 20 steps track the oracle
+    s = scenes.small_block(10, 10, 10, seed=2)
+    s.dt = 1e-4
+    e = _engine(sphk, s, formulation=1)
+    o = _oracle(oracle, s, formulation=1)
+    for k in range(20):
+        e.step(s.dt)
+        o.step(s.dt)
+    np.testing.assert_allclose(e.positions(), o.get(0), rtol=0, atol=2e-6)
+    np.testing.assert_allclose(e.velocities(), o.get(1), rtol=0, atol=2e-3)
+    rho = e.densities()[: s.n]
+    np.testing.assert_allclose(rho, o.get(2), rtol=1e-4)
+    assert e.step_count() == 20
+    assert e.time() == pytest.approx(20 * 1e-4, rel=1e-5)
+
+
+def test_reference_formulation_pileup_neighbors(sphk, oracle, scenes):
+    # reference physics: the first step throws almost every particle into the box corners, so
+    # later steps run on piles of exactly coincident positions.  Neighbour lists of the GPU
+    # state must equal the oracle's hash query on that same state at every step.
+    s = scenes.small_block(8, 8, 8, seed=4)
+    e = _engine(sphk, s)
+    o = oracle.Oracle(s.n, s.oracle_params(oracle))
+    for k in range(4):
+        o.set_particles(e.positions(), e.velocities())
+        o.lib.or_update_neighbor_lists(o.h)
+        _check_neighbors(e, o)
+        e.step(s.dt)
+    pos = e.positions()
+    corners = np.isin(pos[:, 0], [0.0, 1.0]) & np.isin(pos[:, 1], [0.0, 0.4])
+    assert corners.mean() > 0.5
+
+
+def test_adaptive_timestep_matches_oracle(sphk, oracle, scenes):
+    s = scenes.small_block(6, 6, 6, seed=5)
+    s.vel = (np.random.default_rng(0).normal(size=s.vel.shape) * 0.3).astype(np.float32)
+    e = _engine(sphk, s)
+    o = _oracle(oracle, s)
+    dt_o = o.lib.or_compute_cfl_timestep(o.h)
+    t0 = e.time()
+    e.step(0.0)
+    assert e.time() - t0 == pytest.approx(dt_o, rel=0, abs=0)
+    # second step uses |a[0]| of the first
+    o.step(0.0)
+    dt_o2 = o.lib.or_compute_cfl_timestep(o.h)
+    t1 = e.time()
+    e.step(0.0)
+    assert np.float32(e.time() - t1) == pytest.approx(dt_o2, rel=1e-5)
+
+
+def test_total_mass_and_energy(sphk, oracle, scenes):
+    s = scenes.config0(seed=9)
+    e = _engine(sphk, s)
+    o = _oracle(oracle, s)
+    e.step(s.dt)
+    o.step(s.dt)
+    assert e.total_mass() == pytest.approx(o.lib.or_total_mass(o.h), rel=1e-5)
+    # exact invariant: the particle masses never change
+    np.testing.assert_array_equal(e.masses(), np.full(s.n, s.mass, np.float32))
+    me, ee = e.conservation_errors()
+    assert ee == 0.0
+    assert me == pytest.approx(abs(e.total_mass() - s.n * 0.001) / (s.n * 0.001), rel=1e-5)
+    ke = e.total_energy()
+    assert ke == pytest.approx(o.lib.or_total_energy(o.h), rel=1e-3)
+
+
+def test_deterministic_and_snapshot_restore(sphk, scenes):
+    s = scenes.config0(seed=1)
+    e = _engine(sphk, s)
+    e.snapshot_save()
+    e.step(s.dt)
+    a1 = e.positions()
+    r1 = e.densities()
+    e.snapshot_restore()
+    np.testing.assert_array_equal(e.positions(), s.pos)
+    e.step(s.dt)
+    np.testing.assert_array_equal(e.positions(), a1)
+    np.testing.assert_array_equal(e.densities(), r1)
+
+
+def test_empty_and_capacity(sphk):
+    e = sphk.Engine(4, sphk.default_params())
+    e.step(0.001)  # no particles: no-op
+    assert e.step_count() == 0
+    pos = np.zeros((6, 3), np.float32) + 0.1
+    with pytest.raises(sphk.SphkError) as ex:
+        e.upload(pos)
+    assert ex.value.code == -3
+    assert e.size() == 4
+    d = e.densities()
+    assert d.shape == (4,)
+    e.clear()
+    assert e.size() == 0
+    # append semantics (ParticleSystem::add_particles)
+    e.upload(pos[:2])
+    e.upload(pos[:1] + 0.01, offset=2)
+    assert e.size() == 3
+
+
+def test_capacity_length_densities(sphk, scenes):
+    s = scenes.small_block(4, 4, 4)
+    e = sphk.Engine(1000, s.params(sphk))
+    e.upload(s.pos)
+    e.step(s.dt)
+    d = e.densities()
+    assert d.shape == (1000,)
+    assert np.all(d[s.n:] == 0)
+    assert np.all(d[: s.n] > 0)
+
+
+def test_stats(sphk, scenes):
+    s = scenes.config0()
+    e = _engine(sphk, s)
+    e.set_timing(True)
+    e.step(s.dt)
+    st = e.stats()
+    assert st.max_neighbors == 257
+    assert st.total_neighbor_queries == s.n
+    assert st.total_time > 0 and st.density_computation_time > 0
+    e.reset_stats()
+    assert e.stats().max_neighbors == 0
+
+
+def test_config1_fullsize_step0(sphk, oracle, scenes):
+    # the 1M-particle dam break at its real shape: density and acceleration of every particle
+    s = scenes.config1()
+    e = _engine(sphk, s)
+    o = _oracle(oracle, s)
+    _oracle_phases(o)
+    e.set_timing(True)
+    e.step(s.dt)
+    st = e.stats()
+    assert st.max_neighbors == o.lib.or_max_neighbors(o.h)
+    assert st.total_neighbors == o.lib.or_neighbor_total(o.h)
+    params = s.oracle_params(oracle)
+    rho_g, rho_o, rel = _check_density(e, o, s, params, "config1")
+    offs, idx = o.neighbors()
+    mass = np.full(s.n, s.mass, np.float32)
+    scale = oracle.accel_term_scale(s.pos, s.vel, mass, rho_o, offs, idx, params)
+    err = np.linalg.norm(e.accelerations().astype(np.float64) - o.get(4), axis=1) / scale
+    assert err.max() <= ACC_TOL
+    # integration given the GPU's accelerations: bit-exact
+    x1, v1 = oracle.integrate_clamp(s.pos, s.vel, e.accelerations(), s.dt, params)
+    np.testing.assert_array_equal(e.positions(), x1)
