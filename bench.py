@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--config", type=int, default=1, help="scene: 0 (10k), 1 (1M), 2 (9.6M)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default walk launch shapes instead of tuning them first")
     ap.add_argument("--continuous", type=int, default=0,
                     help="also time N continuous steps of the consistent formulation at dt=1e-4")
     return ap.parse_args()
@@ -94,6 +96,11 @@ def main():
     scene = sph_scenes.CONFIGS[args.config]()
     e = sphk.Engine(scene.n, scene.params(sphk))
     e.upload(scene.pos, scene.vel)
+    tune = None
+    if not args.no_autotune:
+        import sph_autotune
+
+        tune = sph_autotune.autotune(e, scene.dt, steps=3, use_cache=False)
     e.snapshot_save()
 
     def one_step():
@@ -152,7 +159,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_force_integrate",
+            "kernel": "k_force_walk",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -170,6 +177,10 @@ def main():
         "phase_ms": {"neighbor_search": t_nbr * 1e3, "density": t_dens * 1e3,
                      "force_integrate": t_force * 1e3, "step_events": st.total_time / k * 1e3},
         "step_hbm_frac": ALG_BYTES["step"] * ups / 1e9 / HBM_PEAK_GBS,
+        "walk_shapes": {"density": sphk.walk_shape_names()[e.walk_shapes()[0]],
+                        "force": sphk.walk_shape_names()[e.walk_shapes()[1]],
+                        "tuning_ms": None if tune is None else
+                        {"density": tune["density_ms"], "force": tune["force_ms"]}},
     }
     if args.continuous:
         e2 = sphk.Engine(scene.n, scene.params(sphk, formulation=1))

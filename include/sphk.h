@@ -163,6 +163,15 @@ int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t 
 /* Grid shape of the last build: origin cell (3), dims (3), number of cells. */
 int sphk_debug_grid(sphk_engine* e, int32_t* origin3, int32_t* dims3, uint64_t* ncells);
 
+/* Launch shapes of the two neighbour walks (LDS words per segment x wavefronts per
+ * workgroup), indexed 0..sphk_walk_shape_count()-1.  Results do not depend on the shape
+ * (each particle's sums run in the same order in every shape); only speed does.  No
+ * reference counterpart: chosen by the autotuner of the `sph` package. */
+int sphk_set_walk_shapes(sphk_engine* e, int density_shape, int force_shape);
+int sphk_get_walk_shapes(const sphk_engine* e, int* density_shape, int* force_shape);
+int sphk_walk_shape_count(void);
+const char* sphk_walk_shape_name(int shape);
+
 /* Last error message of the calling thread; returns its length. */
 int sphk_last_error(char* buf, size_t len);
 /* Library version string ("1.0.0", the reference module's __version__). */

@@ -78,6 +78,11 @@ struct sphk_engine {
     uint32_t* vals = nullptr;
     uint32_t* vals_alt = nullptr;
     uint32_t* hist = nullptr;
+    uint32_t* scan_scratch = nullptr;
+    size_t scan_scratch_cap = 0;
+    WalkBuffers walk;
+    int shape_density = 0;  // walk launch shapes (walk_shape_name), set by sphk_set_walk_shapes
+    int shape_force = 0;
     float4* spm = nullptr;
     float4* svr = nullptr;
     uint32_t* cell_start = nullptr;
@@ -322,6 +327,34 @@ int ensure_grid(sphk_engine* e) {
     e->grid = g;
     e->grid_valid = true;
     e->consts = make_consts(e->params, g);
+    // chunk work lists and neighbour words: at most ceil(C/64) + one partial chunk per column
+    const size_t ncols = (size_t)g.nx * (size_t)g.ny;
+    const size_t chunks = (e->cap + 63) / 64 + ncols;
+    WalkBuffers& wb = e->walk;
+    if (chunks > wb.chunk_cap) {
+        for (uint32_t* p : {wb.items, wb.mask, wb.wj0, wb.nwords})
+            if (p) HIP_TRY(hipFree(p));
+        size_t want = std::max(chunks, wb.chunk_cap * 5 / 4);
+        HIP_TRY(dalloc(&wb.items, want));
+        HIP_TRY(dalloc(&wb.mask, want * kMaxWords * 64));
+        HIP_TRY(dalloc(&wb.wj0, want * kMaxWords));
+        HIP_TRY(dalloc(&wb.nwords, want));
+        wb.chunk_cap = want;
+    }
+    if (ncols + 1 > wb.col_cap) {
+        if (wb.colcnt) HIP_TRY(hipFree(wb.colcnt));
+        size_t want = std::max(ncols + 1, wb.col_cap * 5 / 4);
+        HIP_TRY(dalloc(&wb.colcnt, want));
+        wb.col_cap = want;
+    }
+    if (!wb.nitems) HIP_TRY(dalloc(&wb.nitems, 1));
+    const size_t hist_words = ((size_t)1 << kRadixMaxBits) * ((e->cap + kRadixTile - 1) / kRadixTile);
+    const size_t need = scan_scratch_words(std::max(hist_words, ncols + 1));
+    if (need > e->scan_scratch_cap) {
+        if (e->scan_scratch) HIP_TRY(hipFree(e->scan_scratch));
+        HIP_TRY(dalloc(&e->scan_scratch, need));
+        e->scan_scratch_cap = need;
+    }
     return SPHK_OK;
 }
 
@@ -338,12 +371,14 @@ int build_sorted(sphk_engine* e) {
     HIP_TRY(launch_cell_keys(e->pm, n, e->consts, e->keys, e->uscal, st));
     uint32_t* ko = nullptr;
     uint32_t* vo = nullptr;
-    HIP_TRY(radix_sort_keys(e->keys, e->keys_alt, e->vals, e->vals_alt, e->hist, n,
-                            total_key_bits(e->grid.G), &ko, &vo, st));
+    HIP_TRY(radix_sort_keys(e->keys, e->keys_alt, e->vals, e->vals_alt, e->hist, e->scan_scratch,
+                            n, total_key_bits(e->grid.G), &ko, &vo, st));
     e->skey = ko;
     e->sid = vo;
     HIP_TRY(launch_cell_start(e->skey, n, (uint32_t)e->grid.G, e->cell_start, st));
     HIP_TRY(launch_gather(e->pm, e->vel, e->sid, n, e->spm, e->svr, st));
+    HIP_TRY(launch_chunks(e->cell_start, e->consts, e->walk.colcnt, e->scan_scratch,
+                          e->walk.items, e->walk.nitems, st));
     e->sorted_valid = true;
     return SPHK_OK;
 }
@@ -452,6 +487,10 @@ int sphk_destroy(sphk_engine* e) {
                     e->hist, e->spm,      e->svr,  e->cell_start, e->fscal, e->uscal,
                     e->nstats};
     for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (void* p : {(void*)e->walk.items, (void*)e->walk.nitems, (void*)e->walk.colcnt,
+                    (void*)e->walk.mask, (void*)e->walk.wj0, (void*)e->walk.nwords,
+                    (void*)e->scan_scratch})
         if (p) (void)hipFree(p);
     for (auto* v : {&e->pending, &e->free_sets})
         for (auto& set : *v)
@@ -581,12 +620,12 @@ int sphk_step(sphk_engine* e, float dt) {
     if (adaptive) HIP_TRY(launch_cfl(e->vel, e->acc, n, e->params, e->uscal + 1, e->fscal, st));
     if (int rc = build_sorted(e)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
-    HIP_TRY(launch_density(e->spm, e->svr, e->skey, e->cell_start, n, e->consts,
-                           reinterpret_cast<uint32_t*>(e->nstats), st));
+    HIP_TRY(launch_density_walk(e->walk, e->spm, e->svr, e->skey, e->cell_start, n, e->consts,
+                                e->nstats, e->shape_density, st));
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
-    HIP_TRY(launch_force_integrate(e->spm, e->svr, e->skey, e->sid, e->cell_start, n, e->consts,
-                                   adaptive ? e->fscal : nullptr, dt, e->pm, e->vel, e->acc,
-                                   e->rho, e->prs, st));
+    HIP_TRY(launch_force_walk(e->walk, e->spm, e->svr, e->skey, e->sid, e->cell_start, n,
+                              e->consts, adaptive ? e->fscal : nullptr, dt, e->pm, e->vel, e->acc,
+                              e->rho, e->prs, e->shape_force, st));
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(launch_advance_time(e->fscal + 1, adaptive ? e->fscal : nullptr, dt, st));
     e->sorted_valid = false;
@@ -913,8 +952,9 @@ int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t 
     const size_t n = e->n;
     uint32_t* dcnt = nullptr;
     HIP_TRY(dalloc(&dcnt, n));
-    HIP_TRY(launch_nbr_count(e->spm, e->skey, e->sid, e->cell_start, (uint32_t)n, e->consts, dcnt,
-                             e->stream));
+    HIP_TRY(launch_fill_u32(dcnt, 0u, n, e->stream));
+    HIP_TRY(launch_nbr_lists(e->walk, e->spm, e->skey, e->sid, e->cell_start, (uint32_t)n,
+                             e->consts, 0, dcnt, nullptr, nullptr, e->shape_density, e->stream));
     std::vector<uint32_t> cnt(n);
     if (n) HIP_TRY(hipMemcpyAsync(cnt.data(), dcnt, n * 4, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -935,8 +975,8 @@ int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t 
     HIP_TRY(dalloc(&doffs, n + 1));
     HIP_TRY(dalloc(&didx, (size_t)o));
     HIP_TRY(hipMemcpyAsync(doffs, offs.data(), (n + 1) * 8, hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(launch_nbr_write(e->spm, e->skey, e->sid, e->cell_start, (uint32_t)n, e->consts, doffs,
-                             didx, e->stream));
+    HIP_TRY(launch_nbr_lists(e->walk, e->spm, e->skey, e->sid, e->cell_start, (uint32_t)n,
+                             e->consts, 1, nullptr, doffs, didx, e->shape_density, e->stream));
     if (o) HIP_TRY(hipMemcpyAsync(idx, didx, (size_t)o * 8, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     HIP_TRY(hipFree(doffs));
@@ -960,6 +1000,29 @@ int sphk_debug_grid(sphk_engine* e, int32_t* origin3, int32_t* dims3, uint64_t* 
     }
     if (ncells) *ncells = e->grid.G;
     return SPHK_OK;
+}
+
+int sphk_set_walk_shapes(sphk_engine* e, int density_shape, int force_shape) {
+    if (int rc = check_engine(e)) return rc;
+    if (density_shape < 0 || density_shape >= kNumWalkShapes || force_shape < 0 ||
+        force_shape >= kNumWalkShapes)
+        return set_error(SPHK_E_INVALID, "walk shapes must be in [0, %d)", kNumWalkShapes);
+    e->shape_density = density_shape;
+    e->shape_force = force_shape;
+    return SPHK_OK;
+}
+
+int sphk_get_walk_shapes(const sphk_engine* e, int* density_shape, int* force_shape) {
+    if (int rc = check_engine(e)) return rc;
+    if (density_shape) *density_shape = e->shape_density;
+    if (force_shape) *force_shape = e->shape_force;
+    return SPHK_OK;
+}
+
+int sphk_walk_shape_count(void) { return kNumWalkShapes; }
+
+const char* sphk_walk_shape_name(int shape) {
+    return (shape >= 0 && shape < kNumWalkShapes) ? walk_shape_name(shape) : "";
 }
 
 int sphk_last_error(char* buf, size_t len) {

@@ -49,34 +49,57 @@ constexpr int kRadixItems = 8;  // keys per thread
 constexpr int kRadixTile = kRadixThreads * kRadixItems;
 constexpr int kRadixMaxBits = 11;
 
-// launchers (sphk_kernels.hip); all enqueue on `st`
+// neighbour walks: words of 32 candidates kept in LDS per wave, and per chunk in HBM
+constexpr int kMaxWords = 128;
+constexpr int kNumWalkShapes = 5;  // launch shapes selectable per walk (see walk_shape_name)
+const char* walk_shape_name(int v);
+
+// HBM buffers of the chunked walks (sized for max_chunks())
+struct WalkBuffers {
+    uint32_t* items = nullptr;    // first sorted slot of each chunk
+    uint32_t* nitems = nullptr;   // [1] number of chunks
+    uint32_t* colcnt = nullptr;   // [ncols + 1] per-column chunk counts -> offsets
+    uint32_t* mask = nullptr;     // [chunks][kMaxWords][64] neighbour bits
+    uint32_t* wj0 = nullptr;      // [chunks][kMaxWords] first slot of each word
+    uint32_t* nwords = nullptr;   // [chunks] words of each chunk
+    size_t chunk_cap = 0;
+    size_t col_cap = 0;
+};
+
+// launchers; all enqueue on `st`
+size_t scan_scratch_words(size_t m);
+hipError_t scan_exclusive(uint32_t* data, uint32_t m, uint32_t* scratch, hipStream_t st);
 hipError_t launch_cell_keys(const float4* pm, uint32_t n, const StepConsts& c, uint32_t* keys,
                             uint32_t* err_flag, hipStream_t st);
 hipError_t radix_sort_keys(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
-                           uint32_t* hist, uint32_t n, int total_bits, uint32_t** keys_out,
-                           uint32_t** vals_out, hipStream_t st);
+                           uint32_t* hist, uint32_t* scan_scratch, uint32_t n, int total_bits,
+                           uint32_t** keys_out, uint32_t** vals_out, hipStream_t st);
 size_t radix_hist_words(uint32_t n, int total_bits);
 hipError_t launch_cell_start(const uint32_t* skey, uint32_t n, uint32_t G, uint32_t* cell_start,
                              hipStream_t st);
 hipError_t launch_gather(const float4* pm, const float4* vel, const uint32_t* sid, uint32_t n,
                          float4* spm, float4* svr, hipStream_t st);
-hipError_t launch_density(const float4* spm, float4* svr, const uint32_t* skey,
-                          const uint32_t* cell_start, uint32_t n, const StepConsts& c,
-                          uint32_t* nbr_stats, hipStream_t st);
-hipError_t launch_force_integrate(const float4* spm, const float4* svr, const uint32_t* skey,
-                                  const uint32_t* sid, const uint32_t* cell_start, uint32_t n,
-                                  const StepConsts& c, const float* dt_dev, float dt,
-                                  float4* pm, float4* vel, float4* acc, float* rho, float* prs,
-                                  hipStream_t st);
+uint32_t max_chunks(uint32_t n, const StepConsts& c);
+hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, uint32_t* cnt_scratch,
+                         uint32_t* scan_scratch, uint32_t* items, uint32_t* nitems,
+                         hipStream_t st);
+hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, float4* svr,
+                               const uint32_t* skey, const uint32_t* cell_start, uint32_t n,
+                               const StepConsts& c, unsigned long long* nbr_stats, int shape,
+                               hipStream_t st);
+hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const float4* svr,
+                             const uint32_t* skey, const uint32_t* sid,
+                             const uint32_t* cell_start, uint32_t n, const StepConsts& c,
+                             const float* dt_dev, float dt, float4* pm, float4* vel, float4* acc,
+                             float* rho, float* prs, int shape, hipStream_t st);
+hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint32_t* skey,
+                            const uint32_t* sid, const uint32_t* cell_start, uint32_t n,
+                            const StepConsts& c, int pass, uint32_t* count_by_id,
+                            const int64_t* offsets_by_id, int64_t* idx, int shape,
+                            hipStream_t st);
 hipError_t launch_cfl(const float4* vel, const float4* acc, uint32_t n, const sphk_params& p,
                       uint32_t* scratch_u32, float* dt_dev, hipStream_t st);
 hipError_t launch_advance_time(float* time_dev, const float* dt_dev, float dt, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
-hipError_t launch_nbr_count(const float4* spm, const uint32_t* skey, const uint32_t* sid,
-                            const uint32_t* cell_start, uint32_t n, const StepConsts& c,
-                            uint32_t* count_by_id, hipStream_t st);
-hipError_t launch_nbr_write(const float4* spm, const uint32_t* skey, const uint32_t* sid,
-                            const uint32_t* cell_start, uint32_t n, const StepConsts& c,
-                            const int64_t* offsets_by_id, int64_t* idx, hipStream_t st);
 
 }  // namespace sphk

@@ -1,0 +1,784 @@
+// sphk_walk.hip -- the two neighbour walks of the SPH step (density; forces + integration).
+//
+// Work item ("chunk"): up to 64 consecutive cell-sorted particles of one (x,y) column of
+// the dense grid, one per lane of a 64-wide wavefront.  Within a column the sort order is
+// ascending z cell, so a chunk covers z cells [zmin, zmax] and all of its candidate
+// neighbours lie in the 3x3 neighbouring columns between z cells zmin-1 and zmax+1 (widened
+// to +-2 on an axis when `axis_edge_flags` proves a pair two cells apart possible for some
+// lane of the chunk).  Each column contributes one contiguous range of sorted slots.
+//
+// Phase 1 (test): candidates are walked in the reference's (dx, dy, dz) order, one at a time
+//   for the whole wave, so the candidate index is wave-uniform and its (x, y, z, m) comes
+//   through the scalar cache.  Each lane evaluates the exact FP32 test and sets one bit of
+//   a 32-bit word per 32 candidates.  Words never straddle two columns; `wj0[w]` is the
+//   sorted slot of the first candidate of word w.
+// Phase 2 (evaluate): every lane walks only its own set bits, in word order (= reference
+//   neighbour order), and evaluates the kernel for those pairs.
+// The density pass stores the words of each chunk in HBM (masks, wj0, word count); the force
+// pass reads them back instead of testing again.  Chunks with more than kMaxWords words
+// (pile-ups of thousands of particles per cell) are re-tested by the force pass.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "sphk_internal.h"
+
+namespace sphk {
+
+namespace {
+
+// LDS visibility between lanes of one wavefront: LDS operations of a wave complete in
+// order, so only the compiler must be kept from reordering them.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ float next_up(float x) {
+    if (x == 0.0f) return __int_as_float(1);
+    int i = __float_as_int(x);
+    return __int_as_float(x > 0.0f ? i + 1 : i - 1);
+}
+
+__device__ __forceinline__ float next_down(float x) {
+    if (x == 0.0f) return -__int_as_float(1);
+    int i = __float_as_int(x);
+    return __int_as_float(x > 0.0f ? i - 1 : i + 1);
+}
+
+// (dx*dx + dy*dy) + dz*dz with dx = a - b, every operation rounded once
+// (within_radius_squared / glm::dot, src/spatial_hash.h:70-73).
+__device__ __forceinline__ float dist2_exact(float ax, float ay, float az, float bx, float by,
+                                             float bz) {
+#pragma clang fp contract(off)
+    float dx = ax - bx;
+    float dy = ay - by;
+    float dz = az - bz;
+    float tx = dx * dx;
+    float ty = dy * dy;
+    float tz = dz * dz;
+    return (tx + ty) + tz;
+}
+
+// Can a particle at coordinate x (cell c = floor(fl(x*inv))) have a neighbour in cell c+2
+// (bit 1) or c-2 (bit 0)?  A neighbour needs fl(fl(b-x)^2) <= r2 on this axis alone, and
+// fl(b-x) grows with b, so it suffices to test the float b nearest to x that lands in cell
+// c+2 (resp. c-2).  Exact; conservative (flag set) if the bracket search does not settle.
+__device__ __forceinline__ int axis_edge_flags(float x, float inv, float r2) {
+#pragma clang fp contract(off)
+    float c = floorf(x * inv);
+    int flags = 0;
+    {
+        float t = c + 2.0f;
+        float b = t / inv;
+        int ok = 0;
+        for (int k = 0; k < 6; ++k) {
+            if (b * inv >= t) {
+                ok = 1;
+                break;
+            }
+            b = next_up(b);
+        }
+        if (ok) {
+            for (int k = 0; k < 6; ++k) {
+                float bd = next_down(b);
+                if (bd * inv >= t) {
+                    b = bd;
+                } else {
+                    ok = 2;
+                    break;
+                }
+            }
+        }
+        float d = b - x;
+        if (ok != 2 || d * d <= r2) flags |= 2;
+    }
+    {
+        float t = c - 1.0f;
+        float b = t / inv;
+        int ok = 0;
+        for (int k = 0; k < 6; ++k) {
+            if (b * inv < t) {
+                ok = 1;
+                break;
+            }
+            b = next_down(b);
+        }
+        if (ok) {
+            for (int k = 0; k < 6; ++k) {
+                float bu = next_up(b);
+                if (bu * inv < t) {
+                    b = bu;
+                } else {
+                    ok = 2;
+                    break;
+                }
+            }
+        }
+        float d = x - b;
+        if (ok != 2 || d * d <= r2) flags |= 1;
+    }
+    return flags;
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+constexpr uint32_t kXcds = 8;
+
+// Block index after the XCD swizzle (see grid_for).
+__device__ __forceinline__ uint32_t swizzled_block() {
+    const uint32_t per = gridDim.x / kXcds;
+    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+__device__ __forceinline__ int unii(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Cubic-spline shape without sigma (CubicSplineKernel::W_3d, src/kernels.cpp:58-66):
+// 2/3 - q^2 + q^3/2 on [0,1], (2-q)^3/6 on (1,2], 0 beyond; both pieces are computed and
+// one is selected.
+__device__ __forceinline__ float w_shape(float q) {
+    const float q2 = q * q;
+    const float win = (2.0f / 3.0f - q2) + 0.5f * q2 * q;
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float wout = (1.0f / 6.0f) * t * t * t;
+    return q <= 1.0f ? win : wout;
+}
+
+// Wave-uniform description of one chunk and its candidate columns.
+struct Chunk {
+    uint32_t s0, s_end;   // sorted slots [s0, s_end)
+    int rx, ry;           // column (grid-relative)
+    int dx0, dx1, dy0, dy1;
+    int zlo, zhi;         // candidate z cells (inclusive), grid-relative
+};
+
+// Cursor over the candidate columns, in (dx, dy) lexicographic order.
+struct Cursor {
+    int dx, dy;
+    uint32_t j, end;  // current slot and end of the current column range
+    bool done;
+};
+
+__device__ __forceinline__ void column_range(const Chunk& ch, int dx, int dy,
+                                             const uint32_t* __restrict__ cs, const StepConsts& c,
+                                             uint32_t& a, uint32_t& b) {
+    uint32_t base = ((uint32_t)(ch.rx + dx) * (uint32_t)c.ny + (uint32_t)(ch.ry + dy)) *
+                    (uint32_t)c.nz;
+    a = uni(cs[base + (uint32_t)ch.zlo]);
+    b = uni(cs[base + (uint32_t)ch.zhi + 1u]);
+}
+
+__device__ __forceinline__ void cursor_begin(Cursor& cu, const Chunk& ch,
+                                             const uint32_t* __restrict__ cs, const StepConsts& c) {
+    cu.dx = ch.dx0;
+    cu.dy = ch.dy0;
+    cu.done = false;
+    column_range(ch, cu.dx, cu.dy, cs, c, cu.j, cu.end);
+}
+
+// Moves to the next column when the current one is exhausted; false when all are.
+__device__ __forceinline__ bool cursor_settle(Cursor& cu, const Chunk& ch,
+                                              const uint32_t* __restrict__ cs,
+                                              const StepConsts& c) {
+    while (cu.j >= cu.end) {
+        if (cu.dy < ch.dy1) {
+            ++cu.dy;
+        } else if (cu.dx < ch.dx1) {
+            ++cu.dx;
+            cu.dy = ch.dy0;
+        } else {
+            cu.done = true;
+            return false;
+        }
+        column_range(ch, cu.dx, cu.dy, cs, c, cu.j, cu.end);
+    }
+    return true;
+}
+
+// Appends one test result to a 32-bit word: word = 2*word + (d2 <= r2).  One compare into
+// VCC and one add-with-carry; the first candidate of a word ends up in bit 31.
+__device__ __forceinline__ uint32_t push_bit(uint32_t word, float d2, float r2) {
+    uint32_t out;
+    asm("v_cmp_ge_f32 vcc, %2, %3\n\tv_addc_co_u32 %0, vcc, %1, %1, vcc"
+        : "=v"(out)
+        : "v"(word), "s"(r2), "v"(d2)
+        : "vcc");
+    return out;
+}
+
+// Per-lane summary of one LDS segment: which words hold a neighbour (bit w) and how many
+// neighbours in total.
+struct SegLane {
+    uint32_t nonempty;
+    uint32_t count;
+};
+
+// Phase 1: fills up to kSeg words of pass bits into LDS (masks[w*64 + lane], wj0[w]).
+// Candidate k of a word sits in bit 31-k (words of fewer than 32 candidates are
+// left-aligned).  Returns the number of words written.
+template <int kSeg>
+__device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
+                                              const float4* __restrict__ spm,
+                                              const uint32_t* __restrict__ cs,
+                                              const StepConsts& c, float px, float py, float pz,
+                                              uint32_t* __restrict__ lmask,
+                                              uint32_t* __restrict__ lwj0, uint32_t lane,
+                                              SegLane& sl) {
+    static_assert(kSeg <= 32, "one nonempty bit per word");
+    int w = 0;
+    sl.nonempty = 0;
+    sl.count = 0;
+    while (w < kSeg && cursor_settle(cu, ch, cs, c)) {
+        const uint32_t j0 = uni(cu.j);
+        const uint32_t cnt = uni(min(32u, cu.end - j0));
+        uint32_t word = 0;
+        const float4* __restrict__ pb = spm + j0;  // uniform base: immediate offsets below
+        if (cnt == 32u) {
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                const float4 q = pb[k];
+                word = push_bit(word, dist2_exact(px, py, pz, q.x, q.y, q.z), c.r2);
+            }
+        } else {
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4 q = pb[k];
+                word = push_bit(word, dist2_exact(px, py, pz, q.x, q.y, q.z), c.r2);
+            }
+            word <<= (32u - cnt);
+        }
+        lmask[w * 64 + lane] = word;
+        if (lane == 0) lwj0[w] = j0;
+        sl.nonempty |= (word != 0u ? 1u : 0u) << w;
+        sl.count += (uint32_t)__builtin_popcount(word);
+        cu.j = j0 + cnt;
+        ++w;
+    }
+    return w;
+}
+
+// Builds the wave-uniform chunk description; returns false for an out-of-range item.
+// Lanes past the chunk get NaN coordinates, so every test fails for them.
+__device__ __forceinline__ bool load_chunk(uint32_t item, const uint32_t* __restrict__ items,
+                                           const uint32_t* __restrict__ nitems,
+                                           const uint32_t* __restrict__ skey,
+                                           const uint32_t* __restrict__ cs,
+                                           const float4* __restrict__ spm, const StepConsts& c,
+                                           uint32_t lane, Chunk& ch, uint32_t& s, bool& active,
+                                           float4& pi) {
+    if (item >= uni(nitems[0])) return false;
+    ch.s0 = uni(items[item]);
+    const uint32_t L0 = uni(skey[ch.s0]);
+    const uint32_t col = L0 / (uint32_t)c.nz;
+    const uint32_t col_end = uni(cs[(col + 1u) * (uint32_t)c.nz]);
+    ch.s_end = min(ch.s0 + 64u, col_end);
+    ch.rx = (int)(col / (uint32_t)c.ny);
+    ch.ry = (int)(col % (uint32_t)c.ny);
+    const int zmin = (int)(L0 % (uint32_t)c.nz);
+    const int zmax = (int)(uni(skey[ch.s_end - 1u]) % (uint32_t)c.nz);
+    s = ch.s0 + lane;
+    active = s < ch.s_end;
+    const float qnan = __int_as_float(0x7fc00000);
+    pi = active ? spm[s] : make_float4(qnan, qnan, qnan, 0.0f);
+    int fx = 0, fy = 0, fz = 0;
+    if (active) {
+        fx = axis_edge_flags(pi.x, c.inv_cell, c.r2);
+        fy = axis_edge_flags(pi.y, c.inv_cell, c.r2);
+        fz = axis_edge_flags(pi.z, c.inv_cell, c.r2);
+    }
+    const int fxl = __ballot(fx & 1) != 0, fxh = __ballot(fx & 2) != 0;
+    const int fyl = __ballot(fy & 1) != 0, fyh = __ballot(fy & 2) != 0;
+    const int fzl = __ballot(fz & 1) != 0, fzh = __ballot(fz & 2) != 0;
+    ch.dx0 = unii(-1 - fxl);
+    ch.dx1 = unii(1 + fxh);
+    ch.dy0 = unii(-1 - fyl);
+    ch.dy1 = unii(1 + fyh);
+    ch.zlo = unii(zmin - 1 - fzl);
+    ch.zhi = unii(zmax + 1 + fzh);
+    return true;
+}
+
+// Per-lane walk over the set bits of one segment, in candidate order.  Run exactly
+// SegLane::count times; refills from the next non-empty word by selects (no inner loop).
+// The word index read ahead stays inside the segment (kSeg - 1 when nothing is left).
+template <int kSeg>
+struct BitWalk {
+    uint32_t m = 0, jb = 0, ne = 0;
+    __device__ __forceinline__ uint32_t next(const uint32_t* __restrict__ lmask,
+                                             const uint32_t* __restrict__ lwj0, uint32_t lane) {
+        const uint32_t w = (uint32_t)__builtin_ctz(ne | (1u << (kSeg - 1)));
+        const uint32_t mn = lmask[w * 64 + lane];
+        const uint32_t jn = lwj0[w];
+        const bool need = m == 0u;
+        m = need ? mn : m;
+        jb = need ? jn : jb;
+        ne = need ? (ne & (ne - 1u)) : ne;
+        const uint32_t k = (uint32_t)__builtin_clz(m);
+        m ^= 0x80000000u >> k;
+        return jb + k;
+    }
+};
+
+__device__ __forceinline__ void add_nbr_stats(uint32_t cnt, unsigned long long* stats,
+                                              uint32_t lane) {
+    uint32_t mx = cnt;
+    unsigned long long sm = cnt;
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t o = __shfl_xor(mx, off);
+        mx = o > mx ? o : mx;
+        sm += __shfl_xor(sm, off);
+    }
+    if (lane == 0) {
+        atomicMax(&stats[0], (unsigned long long)mx);
+        atomicAdd(&stats[1], sm);
+    }
+}
+
+// Stores the LDS words [0, nw) of segment `seg_base` of this chunk into HBM.
+__device__ __forceinline__ void store_words(uint32_t item, int seg_base, int nw,
+                                            const uint32_t* __restrict__ lmask,
+                                            const uint32_t* __restrict__ lwj0,
+                                            uint32_t* __restrict__ gmask,
+                                            uint32_t* __restrict__ gwj0, uint32_t lane) {
+    const size_t base = (size_t)item * kMaxWords;
+    for (int w = 0; w < nw; ++w) {
+        int gw = seg_base + w;
+        if (gw >= kMaxWords) break;
+        gmask[(base + gw) * 64 + lane] = lmask[w * 64 + lane];
+    }
+    for (int w = (int)lane; w < nw; w += 64) {
+        int gw = seg_base + w;
+        if (gw < kMaxWords) gwj0[base + gw] = lwj0[w];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// chunk list: per column, ceil(len/64) chunks of consecutive sorted slots
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_chunk_count(const uint32_t* __restrict__ cs, uint32_t ncols,
+                                                     uint32_t nz, uint32_t* __restrict__ cnt) {
+    uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col > ncols) return;
+    if (col == ncols) {
+        cnt[col] = 0;
+        return;
+    }
+    uint32_t a = cs[col * nz], b = cs[(col + 1) * nz];
+    cnt[col] = (b - a + 63u) / 64u;
+}
+
+__global__ __launch_bounds__(256) void k_chunk_fill(const uint32_t* __restrict__ cs, uint32_t ncols,
+                                                    uint32_t nz, const uint32_t* __restrict__ offs,
+                                                    uint32_t* __restrict__ items,
+                                                    uint32_t* __restrict__ nitems) {
+    uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col == 0) nitems[0] = offs[ncols];
+    if (col >= ncols) return;
+    uint32_t a = cs[col * nz], b = cs[(col + 1) * nz];
+    uint32_t o = offs[col];
+    for (uint32_t s = a; s < b; s += 64u) items[o++] = s;
+}
+
+hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, uint32_t* cnt_scratch,
+                         uint32_t* scan_scratch, uint32_t* items, uint32_t* nitems,
+                         hipStream_t st) {
+    const uint32_t ncols = (uint32_t)c.nx * (uint32_t)c.ny;
+    hipLaunchKernelGGL(k_chunk_count, dim3((ncols + 1 + 255) / 256), dim3(256), 0, st, cell_start,
+                       ncols, (uint32_t)c.nz, cnt_scratch);
+    hipError_t e = scan_exclusive(cnt_scratch, ncols + 1, scan_scratch, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_chunk_fill, dim3((ncols + 255) / 256 + 1), dim3(256), 0, st, cell_start,
+                       ncols, (uint32_t)c.nz, cnt_scratch, items, nitems);
+    return hipGetLastError();
+}
+
+uint32_t max_chunks(uint32_t n, const StepConsts& c) {
+    return (n + 63u) / 64u + (uint32_t)c.nx * (uint32_t)c.ny;
+}
+
+// ---------------------------------------------------------------------------
+// Launch shapes of the walks: words per LDS segment x wavefronts per workgroup.  Every
+// wavefront works on its own chunk; the waves of a workgroup never synchronise.
+// ---------------------------------------------------------------------------
+template <int kSeg, int kWaves>
+struct WalkShape {
+    static constexpr int seg = kSeg;
+    static constexpr int waves = kWaves;
+    static constexpr int threads = 64 * kWaves;
+};
+
+// ---------------------------------------------------------------------------
+// density: rho_i = m_i W(0) + sum_{j != i} m_j W(r_ij)   (compute_density,
+// src/sph_engine.cpp:370-383).  Stores the neighbour words of every chunk.
+// ---------------------------------------------------------------------------
+template <class Shape>
+__global__ __launch_bounds__(Shape::threads) void k_density_walk(
+    const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
+    const float4* __restrict__ spm, float4* __restrict__ svr, const uint32_t* __restrict__ skey,
+    const uint32_t* __restrict__ cs, StepConsts c, uint32_t* __restrict__ gmask,
+    uint32_t* __restrict__ gwj0, uint32_t* __restrict__ gnw,
+    unsigned long long* __restrict__ stats) {
+    constexpr int kSeg = Shape::seg;
+    __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
+    __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
+    const uint32_t wave = uni(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* lmask = lmask_all[wave];
+    uint32_t* lwj0 = lwj0_all[wave];
+    const uint32_t item = swizzled_block() * Shape::waves + wave;
+    Chunk ch;
+    uint32_t s;
+    bool active;
+    float4 pi;
+    if (!load_chunk(item, items, nitems, skey, cs, spm, c, lane, ch, s, active, pi)) return;
+    Cursor cu;
+    cursor_begin(cu, ch, cs, c);
+    float acc = 0.0f;
+    uint32_t cnt = 0;
+    int seg_base = 0;
+    for (;;) {
+        SegLane sl;
+        const int nw =
+            phase1_segment<kSeg>(cu, ch, spm, cs, c, pi.x, pi.y, pi.z, lmask, lwj0, lane, sl);
+        if (nw == 0) break;
+        wave_sync();
+        store_words(item, seg_base, nw, lmask, lwj0, gmask, gwj0, lane);
+        cnt += sl.count;
+        BitWalk<kSeg> bw;
+        bw.ne = sl.nonempty;
+        for (uint32_t t = 0; t < sl.count; ++t) {
+            const uint32_t j = bw.next(lmask, lwj0, lane);
+            const float4 q = spm[j];
+            const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
+            const float w = w_shape(__builtin_amdgcn_sqrtf(d2) * c.inv_h);
+            acc += (j != s) ? q.w * w : 0.0f;
+        }
+        seg_base += nw;
+        wave_sync();
+        if (cu.done) break;
+    }
+    if (lane == 0) gnw[item] = (uint32_t)seg_base;
+    if (active) reinterpret_cast<float*>(svr)[4 * s + 3] = pi.w * c.w0 + c.sigma * acc;
+    if (stats) add_nbr_stats(cnt, stats, lane);
+}
+
+// ---------------------------------------------------------------------------
+// forces + leapfrog + wall clamp
+//   F_p = -sum_{j != i, |r| >= 1e-6} m_j (P_i + P_j) / (2 rho_j) gradW(r_ij)
+//   F_v =  sum_{j != i} (m_j / rho_j) mu (v_j - v_i) laplacianW(r_ij)
+//   a_i = (F_p + F_v + g m_i) / m_i        (rho_i instead of m_i: consistent formulation)
+// (src/sph_engine.cpp:390-443), integrate_leapfrog (:290-310) and
+// apply_boundary_conditions (src/particle.cpp:122-153).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void integrate_clamp(float4& p, float3& v, float3 a, float dt,
+                                                const StepConsts& c) {
+#pragma clang fp contract(off)
+    float hx = 0.5f * a.x, hy = 0.5f * a.y, hz = 0.5f * a.z;
+    v.x = v.x + hx * dt;
+    v.y = v.y + hy * dt;
+    v.z = v.z + hz * dt;
+    p.x = p.x + v.x * dt;
+    p.y = p.y + v.y * dt;
+    p.z = p.z + v.z * dt;
+    v.x = v.x + hx * dt;
+    v.y = v.y + hy * dt;
+    v.z = v.z + hz * dt;
+    v.x = v.x * c.damping;
+    v.y = v.y * c.damping;
+    v.z = v.z * c.damping;
+    const float wd = -0.8f;
+    if (p.x < c.lo[0]) {
+        p.x = c.lo[0];
+        v.x = v.x * wd;
+    } else if (p.x > c.hi[0]) {
+        p.x = c.hi[0];
+        v.x = v.x * wd;
+    }
+    if (p.y < c.lo[1]) {
+        p.y = c.lo[1];
+        v.y = v.y * wd;
+    } else if (p.y > c.hi[1]) {
+        p.y = c.hi[1];
+        v.y = v.y * wd;
+    }
+    if (p.z < c.lo[2]) {
+        p.z = c.lo[2];
+        v.z = v.z * wd;
+    } else if (p.z > c.hi[2]) {
+        p.z = c.hi[2];
+        v.z = v.z * wd;
+    }
+}
+
+__device__ __forceinline__ float3 final_accel(float3 fp, float3 fv, float m_i, float rho_i,
+                                              const StepConsts& c) {
+    float fx = fp.x * c.grad_scale + fv.x * c.lap_scale;
+    float fy = fp.y * c.grad_scale + fv.y * c.lap_scale;
+    float fz = fp.z * c.grad_scale + fv.z * c.lap_scale;
+    float m = c.formulation == 1 ? rho_i : m_i;
+    float inv_m = 1.0f / m;
+    return make_float3(fx * inv_m, (fy + c.grav * m) * inv_m, fz * inv_m);
+}
+
+template <class Shape>
+__global__ __launch_bounds__(Shape::threads) void k_force_walk(
+    const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
+    const float4* __restrict__ spm, const float4* __restrict__ svr,
+    const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sid,
+    const uint32_t* __restrict__ cs, StepConsts c, const uint32_t* __restrict__ gmask,
+    const uint32_t* __restrict__ gwj0, const uint32_t* __restrict__ gnw,
+    const float* __restrict__ dt_dev, float dt_arg, float4* __restrict__ pm,
+    float4* __restrict__ vel, float4* __restrict__ acc_out, float* __restrict__ rho_out,
+    float* __restrict__ prs_out) {
+    constexpr int kSeg = Shape::seg;
+    __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
+    __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
+    const uint32_t wave = uni(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* lmask = lmask_all[wave];
+    uint32_t* lwj0 = lwj0_all[wave];
+    const uint32_t item = swizzled_block() * Shape::waves + wave;
+    Chunk ch;
+    uint32_t s;
+    bool active;
+    float4 pi;
+    if (!load_chunk(item, items, nitems, skey, cs, spm, c, lane, ch, s, active, pi)) return;
+    const float4 vi = active ? svr[s] : make_float4(0.f, 0.f, 0.f, 1.f);
+    const float rho_i = vi.w;
+    const float P_i = c.k_gas * (rho_i - c.rho0);
+    float3 fp = make_float3(0.f, 0.f, 0.f);
+    float3 fv = make_float3(0.f, 0.f, 0.f);
+    const int total_words = (int)uni(gnw[item]);
+    const bool stored = total_words <= kMaxWords;
+    Cursor cu;
+    if (!stored) cursor_begin(cu, ch, cs, c);
+    int seg_base = 0;
+    for (;;) {
+        int nw;
+        SegLane sl;
+        if (stored) {
+            nw = min(kSeg, total_words - seg_base);
+            if (nw <= 0) break;
+            const size_t base = (size_t)item * kMaxWords + seg_base;
+            sl.nonempty = 0;
+            sl.count = 0;
+            for (int w = 0; w < nw; ++w) {
+                const uint32_t word = gmask[(base + w) * 64 + lane];
+                lmask[w * 64 + lane] = word;
+                sl.nonempty |= (word != 0u ? 1u : 0u) << w;
+                sl.count += (uint32_t)__builtin_popcount(word);
+            }
+            for (int w = (int)lane; w < nw; w += 64) lwj0[w] = gwj0[base + w];
+        } else {
+            nw = phase1_segment<kSeg>(cu, ch, spm, cs, c, pi.x, pi.y, pi.z, lmask, lwj0, lane, sl);
+            if (nw == 0) break;
+        }
+        wave_sync();
+        BitWalk<kSeg> bw;
+        bw.ne = sl.nonempty;
+        for (uint32_t t = 0; t < sl.count; ++t) {
+            const uint32_t j = bw.next(lmask, lwj0, lane);
+            const float4 pj = spm[j];
+            const float4 vj = svr[j];
+            const float d2 = dist2_exact(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
+            const float rl = __builtin_amdgcn_sqrtf(d2);
+            const float q = rl * c.inv_h;
+            const float t2 = fmaxf(2.0f - q, 0.0f);
+            const float inv_rho_j = __builtin_amdgcn_rcpf(vj.w);
+            const bool inner = q <= 1.0f;
+            const float dwdq = inner ? q * (-2.0f + 1.5f * q) : -0.5f * t2 * t2;
+            const float lapf = inner ? (-2.0f + 3.0f * q) : t2;
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            const bool other = j != s;
+            const float P_j = c.k_gas * (vj.w - c.rho0);
+            // pressure: skipped for |r| < 1e-6 (and self); viscosity: skipped for self only
+            const float coef = (other && rl >= 1e-6f)
+                                   ? pj.w * (P_i + P_j) * 0.5f * inv_rho_j * dwdq *
+                                         __builtin_amdgcn_rcpf(rl)
+                                   : 0.0f;
+            fp.x -= coef * rx;
+            fp.y -= coef * ry;
+            fp.z -= coef * rz;
+            const float cv = other ? pj.w * inv_rho_j * lapf : 0.0f;
+            fv.x += cv * (vj.x - vi.x);
+            fv.y += cv * (vj.y - vi.y);
+            fv.z += cv * (vj.z - vi.z);
+        }
+        seg_base += nw;
+        wave_sync();
+        if (!stored && cu.done) break;
+    }
+    if (!active) return;
+    const float3 a = final_accel(fp, fv, pi.w, rho_i, c);
+    const float dt = dt_dev ? *dt_dev : dt_arg;
+    float3 v = make_float3(vi.x, vi.y, vi.z);
+    float4 p = pi;
+    integrate_clamp(p, v, a, dt, c);
+    const uint32_t id = sid[s];
+    pm[id] = p;
+    vel[id] = make_float4(v.x, v.y, v.z, 0.0f);
+    acc_out[id] = make_float4(a.x, a.y, a.z, 0.0f);
+    rho_out[id] = rho_i;
+    prs_out[id] = P_i;
+}
+
+// Particles with a non-finite coordinate (dense index G) have no neighbour, not even
+// themselves: rho = m W(0), a = (0, g, 0); they are integrated and clamped like the rest.
+__global__ __launch_bounds__(256) void k_nonfinite_step(
+    const float4* __restrict__ spm, float4* __restrict__ svr, const uint32_t* __restrict__ sid,
+    const uint32_t* __restrict__ cs, uint32_t n, StepConsts c, const float* __restrict__ dt_dev,
+    float dt_arg, float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc_out,
+    float* __restrict__ rho_out, float* __restrict__ prs_out) {
+    const uint32_t first = cs[c.G];
+    const uint32_t s = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float4 pi = spm[s];
+    const float4 vi = svr[s];
+    const float rho_i = pi.w * c.w0;
+    const float P_i = c.k_gas * (rho_i - c.rho0);
+    const float3 a = final_accel(make_float3(0.f, 0.f, 0.f), make_float3(0.f, 0.f, 0.f), pi.w,
+                                 rho_i, c);
+    const float dt = dt_dev ? *dt_dev : dt_arg;
+    float3 v = make_float3(vi.x, vi.y, vi.z);
+    float4 p = pi;
+    integrate_clamp(p, v, a, dt, c);
+    const uint32_t id = sid[s];
+    pm[id] = p;
+    vel[id] = make_float4(v.x, v.y, v.z, 0.0f);
+    acc_out[id] = make_float4(a.x, a.y, a.z, 0.0f);
+    rho_out[id] = rho_i;
+    prs_out[id] = P_i;
+}
+
+// ---------------------------------------------------------------------------
+// neighbour lists for parity checks: the production phase-1 test, expanded to CSR over
+// caller ids in reference order.  pass 0 counts, pass 1 writes.
+// ---------------------------------------------------------------------------
+template <class Shape>
+__global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
+    const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
+    const float4* __restrict__ spm, const uint32_t* __restrict__ skey,
+    const uint32_t* __restrict__ sid, const uint32_t* __restrict__ cs, StepConsts c, int pass,
+    uint32_t* __restrict__ cnt_by_id, const int64_t* __restrict__ offs_by_id,
+    int64_t* __restrict__ idx) {
+    constexpr int kSeg = Shape::seg;
+    __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
+    __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
+    const uint32_t wave = uni(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* lmask = lmask_all[wave];
+    uint32_t* lwj0 = lwj0_all[wave];
+    Chunk ch;
+    uint32_t s;
+    bool active;
+    float4 pi;
+    if (!load_chunk(swizzled_block() * Shape::waves + wave, items, nitems, skey, cs, spm, c, lane, ch, s,
+                    active, pi))
+        return;
+    Cursor cu;
+    cursor_begin(cu, ch, cs, c);
+    uint32_t cnt = 0;
+    int64_t o = (pass == 1 && active) ? offs_by_id[sid[s]] : 0;
+    for (;;) {
+        SegLane sl;
+        const int nw =
+            phase1_segment<kSeg>(cu, ch, spm, cs, c, pi.x, pi.y, pi.z, lmask, lwj0, lane, sl);
+        if (nw == 0) break;
+        wave_sync();
+        if (pass == 1) {
+            BitWalk<kSeg> bw;
+            bw.ne = sl.nonempty;
+            for (uint32_t t = 0; t < sl.count; ++t) idx[o++] = sid[bw.next(lmask, lwj0, lane)];
+        }
+        cnt += sl.count;
+        wave_sync();
+        if (cu.done) break;
+    }
+    if (pass == 0 && active) cnt_by_id[sid[s]] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// launch shape table (index = WalkTuning value)
+// ---------------------------------------------------------------------------
+using Shape0 = WalkShape<16, 4>;
+using Shape1 = WalkShape<32, 4>;
+using Shape2 = WalkShape<32, 2>;
+using Shape3 = WalkShape<32, 1>;
+using Shape4 = WalkShape<8, 4>;
+
+const char* walk_shape_name(int v) {
+    switch (v) {
+        case 0: return "seg16_waves4";
+        case 1: return "seg32_waves4";
+        case 2: return "seg32_waves2";
+        case 3: return "seg32_waves1";
+        case 4: return "seg8_waves4";
+        default: return "?";
+    }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The grid is
+// padded to a multiple of 8 and block b works on block (b % 8) * (grid / 8) + b / 8, so each
+// XCD walks one contiguous run of chunks, whose candidate columns overlap.
+template <class Shape>
+static uint32_t grid_for(uint32_t chunks) {
+    uint32_t blocks = (chunks + Shape::waves - 1) / Shape::waves;
+    return (blocks + kXcds - 1) / kXcds * kXcds;
+}
+
+#define SPHK_DISPATCH_SHAPE(v, CALL)     \
+    switch (v) {                         \
+        case 0: { using S = Shape0; CALL; break; } \
+        case 1: { using S = Shape1; CALL; break; } \
+        case 2: { using S = Shape2; CALL; break; } \
+        case 3: { using S = Shape3; CALL; break; } \
+        default: { using S = Shape4; CALL; break; } \
+    }
+
+hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, float4* svr,
+                               const uint32_t* skey, const uint32_t* cell_start, uint32_t n,
+                               const StepConsts& c, unsigned long long* nbr_stats, int shape,
+                               hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t chunks = max_chunks(n, c);
+    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_density_walk<S>, dim3(grid_for<S>(chunks)),
+                                                  dim3(S::threads), 0, st, wb.items, wb.nitems,
+                                                  spm, svr, skey, cell_start, c, wb.mask, wb.wj0,
+                                                  wb.nwords, nbr_stats));
+    return hipGetLastError();
+}
+
+hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const float4* svr,
+                             const uint32_t* skey, const uint32_t* sid,
+                             const uint32_t* cell_start, uint32_t n, const StepConsts& c,
+                             const float* dt_dev, float dt, float4* pm, float4* vel, float4* acc,
+                             float* rho, float* prs, int shape, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t chunks = max_chunks(n, c);
+    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_force_walk<S>, dim3(grid_for<S>(chunks)),
+                                                  dim3(S::threads), 0, st, wb.items, wb.nitems,
+                                                  spm, svr, skey, sid, cell_start, c, wb.mask,
+                                                  wb.wj0, wb.nwords, dt_dev, dt, pm, vel, acc, rho,
+                                                  prs));
+    // the non-finite bin holds at most n particles; the kernel reads its start on the device
+    hipLaunchKernelGGL(k_nonfinite_step, dim3((n + 255) / 256), dim3(256), 0, st, spm,
+                       const_cast<float4*>(svr), sid, cell_start, n, c, dt_dev, dt, pm, vel, acc,
+                       rho, prs);
+    return hipGetLastError();
+}
+
+hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint32_t* skey,
+                            const uint32_t* sid, const uint32_t* cell_start, uint32_t n,
+                            const StepConsts& c, int pass, uint32_t* count_by_id,
+                            const int64_t* offsets_by_id, int64_t* idx, int shape,
+                            hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t chunks = max_chunks(n, c);
+    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_nbr_lists<S>, dim3(grid_for<S>(chunks)),
+                                                  dim3(S::threads), 0, st, wb.items, wb.nitems,
+                                                  spm, skey, sid, cell_start, c, pass, count_by_id,
+                                                  offsets_by_id, idx));
+    return hipGetLastError();
+}
+
+}  // namespace sphk

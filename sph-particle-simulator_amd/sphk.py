@@ -92,9 +92,15 @@ SIGNATURES = {
     "sphk_debug_sorted": [_P, _U32, _U32, _SZ],
     "sphk_debug_neighbors": [_P, _I64, _I64, _SZ, C.POINTER(_SZ)],
     "sphk_debug_grid": [_P, _I32, _I32, _U64],
+    "sphk_set_walk_shapes": [_P, C.c_int, C.c_int],
+    "sphk_get_walk_shapes": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "sphk_walk_shape_count": [],
+    "sphk_walk_shape_name": [C.c_int],
     "sphk_last_error": [C.c_char_p, _SZ],
     "sphk_version": [],
 }
+
+_STRING_RESULTS = {"sphk_version", "sphk_walk_shape_name"}
 
 _lib: Optional[C.CDLL] = None
 
@@ -110,7 +116,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = C.c_char_p if name == "sphk_version" else C.c_int
+        fn.restype = C.c_char_p if name in _STRING_RESULTS else C.c_int
     _lib = lib
     return lib
 
@@ -262,6 +268,14 @@ class Engine:
     def snapshot_restore(self):
         check(self.lib.sphk_snapshot_restore(self.h))
 
+    def set_walk_shapes(self, density: int, force: int):
+        check(self.lib.sphk_set_walk_shapes(self.h, int(density), int(force)))
+
+    def walk_shapes(self):
+        a, b = C.c_int(), C.c_int()
+        check(self.lib.sphk_get_walk_shapes(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     # ---- parity/debug ----
     def cell_coords(self) -> np.ndarray:
         n = self.size()
@@ -302,3 +316,8 @@ class Engine:
 
 def version() -> str:
     return load().sphk_version().decode()
+
+
+def walk_shape_names():
+    lib = load()
+    return [lib.sphk_walk_shape_name(i).decode() for i in range(lib.sphk_walk_shape_count())]

@@ -323,3 +323,50 @@ def test_config1_fullsize_step0(sphk, oracle, scenes):
     # integration given the GPU's accelerations: bit-exact
     x1, v1 = oracle.integrate_clamp(s.pos, s.vel, e.accelerations(), s.dt, params)
     np.testing.assert_array_equal(e.positions(), x1)
+
+
+def test_walk_shapes_bitexact(sphk, scenes):
+    # every launch shape of the walks gives bit-identical results (same per-particle order)
+    s = scenes.config0(seed=21)
+    ref = None
+    for v in range(len(sphk.walk_shape_names())):
+        e = _engine(sphk, s)
+        e.set_walk_shapes(v, v)
+        e.step(s.dt)
+        out = (e.positions(), e.velocities(), e.densities(), e.accelerations())
+        if ref is None:
+            ref = out
+        else:
+            for a, b in zip(ref, out):
+                np.testing.assert_array_equal(a, b)
+        offs, idx = e.neighbors()
+        if v == 0:
+            ref_n = (offs, idx)
+        else:
+            np.testing.assert_array_equal(offs, ref_n[0])
+            np.testing.assert_array_equal(idx, ref_n[1])
+
+
+def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes):
+    # a pile-up cell with more than 4096 candidates per chunk exceeds the stored-word budget;
+    # the force pass then re-tests.  Both paths must agree with the oracle.
+    rng = np.random.default_rng(3)
+    pos = np.concatenate([rng.uniform(0.1, 0.13, size=(3000, 3)),
+                          rng.uniform(0.3, 0.5, size=(500, 3))]).astype(np.float32)
+    p = sphk.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
+    e = sphk.Engine(len(pos), p)
+    e.upload(pos)
+    o = oracle.Oracle(len(pos), oracle.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0,
+                                                      zmax=1))
+    o.set_particles(pos)
+    _oracle_phases(o)
+    _check_neighbors(e, o)
+    e.step(0.001)
+    rho = e.densities()[: len(pos)]
+    np.testing.assert_allclose(rho, o.get(2), rtol=1e-5)
+    offs, idx = o.neighbors()
+    mass = np.full(len(pos), 0.001, np.float32)
+    scale = oracle.accel_term_scale(pos, np.zeros_like(pos), mass, o.get(2), offs, idx,
+                                    o.params)
+    err = np.linalg.norm(e.accelerations().astype(np.float64) - o.get(4), axis=1) / scale
+    assert err.max() <= ACC_TOL
