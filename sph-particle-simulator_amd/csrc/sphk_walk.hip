@@ -576,10 +576,18 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
         wave_sync();
         BitWalk<kSeg> bw;
         bw.ne = sl.nonempty;
+        // neighbour data is requested one iteration ahead; the look-ahead past the last
+        // neighbour re-reads the current slot
+        uint32_t jn = sl.count ? bw.next(lmask, lwj0, lane) : 0u;
+        float4 pn = spm[jn];
+        float4 vn = svr[jn];
         for (uint32_t t = 0; t < sl.count; ++t) {
-            const uint32_t j = bw.next(lmask, lwj0, lane);
-            const float4 pj = spm[j];
-            const float4 vj = svr[j];
+            const uint32_t j = jn;
+            const float4 pj = pn;
+            const float4 vj = vn;
+            if (t + 1 < sl.count) jn = bw.next(lmask, lwj0, lane);
+            pn = spm[jn];
+            vn = svr[jn];
             const float d2 = dist2_exact(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
             const float rl = __builtin_amdgcn_sqrtf(d2);
             const float q = rl * c.inv_h;
