@@ -606,6 +606,10 @@ int sphk_step(sphk_engine* e, float dt) {
     const uint32_t n = (uint32_t)e->n;
     const bool adaptive = !(dt > 0.0f);
     hipEvent_t* ev = nullptr;
+    // bound the events waiting to be read: fold the oldest in every 256 timed steps
+    if (e->timing && e->pending.size() >= 256) {
+        if (int rc = drain_timing(e)) return rc;
+    }
     if (e->timing) {
         if (e->free_sets.empty()) {
             sphk_engine::EventSet set;

@@ -1,0 +1,170 @@
+"""The `sph` module (pybind11 over the C++ SPHEngine) as a drop-in for the reference module
+built from python/bindings.cpp.  CPU tests cover the API surface and scene builders; GPU
+tests step the engine and compare with the oracle."""
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def sph():
+    import sph as mod
+
+    return mod
+
+
+def test_module_surface(sph):
+    assert sph.__version__ == "1.0.0"
+    for name in ("Simulator", "SPHParameters", "Particle", "ParticleType", "ParticleSystem",
+                 "PerformanceStats", "create_fluid_block", "create_boundary_box"):
+        assert hasattr(sph, name)
+    for meth in ("initialize", "initialize_dam_break", "initialize_fluid_drop",
+                 "initialize_granular_flow", "add_particles", "clear_particles", "step",
+                 "run_steps", "get_particles", "get_parameters", "get_current_time",
+                 "get_step_count", "set_parameters", "set_gravity", "set_viscosity",
+                 "set_smoothing_length", "set_boundaries", "get_performance_stats",
+                 "reset_performance_stats", "compute_conservation_errors", "get_total_mass",
+                 "get_total_energy", "validate_simulation", "is_initialized", "get_positions",
+                 "get_velocities", "get_densities", "get_pressures"):
+        assert hasattr(sph.Simulator, meth), meth
+
+
+def test_parameters_defaults(sph):
+    p = sph.SPHParameters()
+    assert (p.rest_density, p.gas_constant, p.smoothing_length, p.particle_mass) == (
+        1000.0, 2000.0, pytest.approx(0.02), pytest.approx(0.001))
+    assert p.timestep == pytest.approx(0.001) and p.gravity == pytest.approx(-9.81)
+    assert p.damping == pytest.approx(0.99) and p.CFL_factor == pytest.approx(0.4)
+    assert p.neighbor_search_radius == pytest.approx(0.04) and p.formulation == 0
+    assert (p.bounds.xmin, p.bounds.xmax) == (-1.0, 1.0)
+
+
+def test_particle_defaults_and_vec3(sph):
+    q = sph.Particle()
+    assert q.mass == 1.0 and q.type == sph.ParticleType.FLUID and q.id == -1
+    assert q.temperature == pytest.approx(293.15)
+    q.position = [1.0, 2.0, 3.0]
+    np.testing.assert_array_equal(q.position, np.array([1, 2, 3], np.float32))
+    r = sph.Particle([0.1, 0.2, 0.3], 0.5, sph.ParticleType.BOUNDARY)
+    assert r.type == sph.ParticleType.BOUNDARY and r.mass == 0.5
+
+
+def test_create_fluid_block(sph):
+    ps = sph.create_fluid_block(np.zeros(3, np.float32), np.array([0.05, 0.04, 0.03], np.float32),
+                                0.01, 0.001)
+    # int(size/spacing) per axis in FP32, x-major order
+    f = np.float32
+    n = [int(f(0.05) / f(0.01)), int(f(0.04) / f(0.01)), int(f(0.03) / f(0.01))]
+    assert len(ps) == n[0] * n[1] * n[2]
+    start = np.zeros(3, np.float32) - np.array([0.05, 0.04, 0.03], np.float32) * f(0.5)
+    np.testing.assert_allclose(ps[0].position, start)
+    # k (z) is the innermost loop
+    np.testing.assert_allclose(ps[1].position - ps[0].position, [0, 0, 0.01], atol=1e-7)
+    assert all(p.type == sph.ParticleType.FLUID and p.mass == pytest.approx(0.001) for p in ps)
+    with pytest.raises(Exception):
+        sph.create_fluid_block(np.zeros(2, np.float32), np.ones(3, np.float32), 0.1)
+
+
+def test_create_boundary_box_faces(sph):
+    ps = sph.create_boundary_box(np.zeros(3, np.float32), np.array([0.4, 0.6, 0.8], np.float32),
+                                 0.01, 0.001)
+    f = np.float32
+    nx, ny, nz = (int(f(0.4) / f(0.01)), int(f(0.6) / f(0.01)), int(f(0.8) / f(0.01)))
+    assert len(ps) == 2 * nx * ny + 2 * ny * nz + 2 * nx * nz
+    assert all(p.type == sph.ParticleType.BOUNDARY for p in ps)
+
+
+def test_dam_break_particle_count(sph):
+    # initialize_dam_break's scene: boundary box (0.4,0.6,0.8) centred (0,0.3,0) + fluid block
+    # (0.2,0.4,0.8): 20,800 boundary + 64,000 fluid = 84,800 particles
+    b = sph.create_boundary_box(np.array([0, 0.3, 0], np.float32), np.array([0.4, 0.6, 0.8], np.float32), 0.01)
+    fl = sph.create_fluid_block(np.array([-0.1, 0.2, 0], np.float32), np.array([0.2, 0.4, 0.8], np.float32), 0.01)
+    assert len(b) == 20800 and len(fl) == 64000
+
+
+def test_simulator_without_gpu_raises(sph):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(RuntimeError):
+        sph.Simulator(100)
+
+
+# ---------------------------------------------------------------- GPU
+@pytest.mark.gpu
+def test_dam_break_step_matches_oracle(sph, oracle):
+    sim = sph.Simulator(100000)
+    assert not sim.is_initialized()
+    sim.step(0.001)  # silent no-op before initialisation
+    sim.initialize_dam_break()
+    assert sim.is_initialized()
+    ps = sim.get_particles()
+    n = ps.size()
+    assert n == 84800 and ps.capacity() == 100000
+    pos0 = sim.get_positions()
+    vel0 = sim.get_velocities()
+    masses = np.array([ps[i].mass for i in range(0, n, 1000)])
+    assert np.all(masses == np.float32(0.001))
+    o = oracle.Oracle(n, oracle.default_params())
+    o.set_particles(pos0, vel0)
+    o.lib.or_update_neighbor_lists(o.h)
+    o.lib.or_compute_densities(o.h)
+    sim.step(0.001)
+    rho = sim.get_densities()
+    assert rho.shape == (100000,)
+    ro = o.get(oracle.OR_DENSITY)
+    np.testing.assert_allclose(rho[:n], ro, rtol=1e-5)
+    assert np.all(rho[n:] == 0)
+    assert sim.get_step_count() == 1
+    assert sim.get_current_time() == pytest.approx(0.001)
+    # the host mirror is refreshed
+    ps = sim.get_particles()
+    np.testing.assert_array_equal(ps[17].position, sim.get_positions()[17])
+    assert ps[17].density == rho[17]
+    st = sim.get_performance_stats()
+    assert st.max_neighbors > 0 and st.total_neighbor_queries == n and st.total_time > 0
+    me, ee = sim.compute_conservation_errors()
+    assert ee == 0.0 and me >= 0.0
+
+
+@pytest.mark.gpu
+def test_add_clear_and_parameters(sph):
+    sim = sph.Simulator(10)
+    p = sph.SPHParameters()
+    p.smoothing_length = 0.03
+    sim.initialize(p)
+    assert sim.get_parameters().neighbor_search_radius == pytest.approx(0.04)  # initialize keeps it
+    sim.set_parameters(p)
+    assert sim.get_parameters().neighbor_search_radius == pytest.approx(0.06)  # re-derived 2h
+    parts = [sph.Particle([0.1 * i, 0.0, 0.0], 0.001) for i in range(12)]
+    sim.add_particles(parts)  # two are dropped: capacity 10
+    assert sim.get_particles().size() == 10
+    assert [sim.get_particles()[i].id for i in range(10)] == list(range(10))
+    sim.step(0.001)
+    sim.clear_particles()
+    assert sim.get_particles().size() == 0 and sim.get_step_count() == 0
+    assert sim.get_current_time() == 0.0
+    sim.add_particles(parts[:3])
+    sim.run_steps(2, False)
+    assert sim.get_step_count() == 2
+    assert sim.get_current_time() == pytest.approx(0.002)
+    sim.set_boundaries(0, 1, 0, 1, 0, 1)
+    sim.set_gravity(-1.0)
+    sim.set_viscosity(0.01)
+    assert sim.get_parameters().bounds.xmax == 1.0 and sim.get_parameters().gravity == -1.0
+
+
+@pytest.mark.gpu
+def test_fluid_drop_and_granular_scenes(sph):
+    sim = sph.Simulator(200000)
+    sim.initialize_fluid_drop()
+    n_drop = sim.get_particles().size()
+    assert n_drop > 0
+    sim.run_steps(2, True)  # adaptive dt
+    assert sim.get_current_time() > 0
+    sim.initialize_granular_flow()
+    ps = sim.get_particles()
+    kinds = {ps[i].type for i in range(ps.size())}
+    assert sph.ParticleType.SOLID in kinds and sph.ParticleType.BOUNDARY in kinds
+    sim.step(0.0005)
+    assert np.isfinite(sim.get_total_energy())
