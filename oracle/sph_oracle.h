@@ -19,8 +19,17 @@
  * plus an FP64 evaluation of density and acceleration over the same neighbour
  * lists (the error yardstick for the FP32 paths).
  *
- * Pinning status of each function is stated in oracle/README.md and is checked by
- * tests/test_oracle.py (closed forms, brute force, lattice invariants).
+ * The reference ships no tests or golden vectors for this path and cannot be compiled here
+ * (its sources need GLM, absent), so every function is pinned by tests/test_oracle.py:
+ *   grid coords / keys    closed form on random points, specials (NaN, negatives, rounding)
+ *   neighbour query       brute force O(n^2), symmetry, lexicographic order, the FP32 pair
+ *                         two cells apart that a 27-cell scan would miss
+ *   W, gradW, laplacianW  closed forms, FP64 finite differences, integral 2/3
+ *   density               perfect-lattice bulk value (FP64 666.7195) and neighbour count 257
+ *   forces                FP64 lattice interior a = g, pressure-term sign, viscosity sign
+ *   leapfrog, clamp, CFL  closed forms
+ * Unpinned: the FP32 bit patterns the reference binary would give under -ffast-math; the
+ * restatement follows the source in IEEE arithmetic.
  */
 #ifndef SPH_ORACLE_H
 #define SPH_ORACLE_H
