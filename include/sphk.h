@@ -104,6 +104,21 @@ int sphk_get_params(const sphk_engine* e, sphk_params* out);
  * (src/particle.cpp:26-33). */
 int sphk_upload(sphk_engine* e, size_t offset, size_t n, const float* pos_xyz,
                 const float* vel_xyz, const float* mass);
+/* Batched scenes.  Splits the particles (caller order) into nscenes independent scenes:
+ * scene k holds particles [offsets[k], offsets[k+1]) and is clamped to its own walls
+ * bounds[6k .. 6k+5] = (xmin, xmax, ymin, ymax, zmin, zmax) (NULL: params' bounds for all).
+ * offsets[0] == 0 and offsets[nscenes] must equal the particle count when stepping.  No
+ * neighbour search crosses two scenes; each scene gets its own dense cell grid, and one step
+ * runs every scene in the same launches.  nscenes == 0 returns to one scene holding every
+ * particle.  No reference counterpart (the reference runs one scene per engine). */
+int sphk_set_scenes(sphk_engine* e, size_t nscenes, const uint64_t* offsets, const float* bounds);
+int sphk_scene_count(const sphk_engine* e, size_t* out);
+
+/* Steps are captured into a HIP graph and replayed while the step's shape (particle count,
+ * grids, launch shapes, dt mode) is unchanged.  Steps with timing enabled run eagerly.
+ * No reference counterpart. */
+int sphk_set_graph(sphk_engine* e, int enabled);
+
 /* Particle count (ParticleSystem::size). */
 int sphk_size(const sphk_engine* e, size_t* out);
 /* Resets the particle count, simulated time and step counter

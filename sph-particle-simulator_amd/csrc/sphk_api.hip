@@ -49,6 +49,26 @@ hipError_t dalloc(T** p, size_t count) {
 
 }  // namespace
 
+// Host view of one scene: particle range, wall bounds, the box its finite positions may
+// occupy, and its dense grid.
+struct SceneHost {
+    uint64_t begin = 0, end = 0;
+    float bounds[6] = {-1.0f, 1.0f, -1.0f, 1.0f, -1.0f, 1.0f};  // xmin xmax ymin ymax zmin zmax
+    float ext_lo[3] = {0, 0, 0}, ext_hi[3] = {0, 0, 0};
+    bool ext_any = false;
+    Grid grid;
+};
+
+// Everything a captured step depends on besides buffer contents.
+struct GraphKey {
+    StepConsts consts;
+    uint32_t n;
+    int32_t shape_density, shape_force;
+    int32_t adaptive;
+    float dt;
+    uint64_t buffer_gen;
+};
+
 struct sphk_engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -69,8 +89,8 @@ struct sphk_engine {
     float4* snap_acc = nullptr;
     size_t snap_n = 0;
     uint64_t snap_steps = 0;
-    float snap_ext_lo[3] = {0, 0, 0}, snap_ext_hi[3] = {0, 0, 0};
-    bool snap_ext_any = false, snap_ext_unknown = false;
+    std::vector<SceneHost> snap_scenes;
+    bool snap_ext_unknown = false;
     bool has_snapshot = false;
     // work arrays
     uint32_t* keys = nullptr;
@@ -96,14 +116,23 @@ struct sphk_engine {
     // [0] max neighbours, [1] total neighbours
     unsigned long long* nstats = nullptr;
 
-    // grid
-    Grid grid;
+    // scenes and their grids
+    std::vector<SceneHost> scenes;  // one implicit scene [0, n) unless set by sphk_set_scenes
+    bool scenes_explicit = false;
+    bool ext_unknown = false;  // positions may lie outside the scene extents (device bbox needed)
     bool grid_valid = false;
-    float ext_lo[3] = {0, 0, 0}, ext_hi[3] = {0, 0, 0};
-    bool ext_any = false;      // ext holds at least one finite position
-    bool ext_unknown = false;  // positions may lie outside ext (needs a device bbox)
+    std::vector<SceneGrid> table;  // host copy of the device scene table
+    SceneGrid* d_scenes = nullptr;
+    size_t d_scenes_cap = 0;
     StepConsts consts{};
     bool sorted_valid = false;  // work arrays hold the cell-sorted image of the current state
+    uint64_t buffer_gen = 0;    // bumped when a buffer used by the step is reallocated
+
+    // captured step
+    bool use_graph = false;
+    bool graph_valid = false;
+    GraphKey graph_key{};
+    hipGraphExec_t graph_exec = nullptr;
 
     // timing: one event set per timed step, drained when stats are read
     struct EventSet {
@@ -137,7 +166,7 @@ void fill_default(sphk_params* p) {
 }
 
 // Host-side constants, in the reference's FP32 expression order.
-StepConsts make_consts(const sphk_params& p, const Grid& g) {
+StepConsts make_consts(const sphk_params& p) {
 #pragma clang fp contract(off)
     StepConsts c{};
     const float pi_f = 3.14159265358979323846f;
@@ -160,20 +189,7 @@ StepConsts make_consts(const sphk_params& p, const Grid& g) {
     c.rho0 = p.rest_density;
     c.grav = p.gravity;
     c.damping = p.damping;
-    c.lo[0] = p.xmin;
-    c.hi[0] = p.xmax;
-    c.lo[1] = p.ymin;
-    c.hi[1] = p.ymax;
-    c.lo[2] = p.zmin;
-    c.hi[2] = p.zmax;
     c.formulation = p.formulation;
-    c.ox = g.ox;
-    c.oy = g.oy;
-    c.oz = g.oz;
-    c.nx = g.nx;
-    c.ny = g.ny;
-    c.nz = g.nz;
-    c.G = (uint32_t)g.G;
     return c;
 }
 
@@ -185,40 +201,47 @@ int floor_cell(float x, float inv, int64_t* out) {
     return 0;
 }
 
-void ext_add_box(sphk_engine* e, const float lo[3], const float hi[3]) {
+void ext_add_box(SceneHost& sc, const float lo[3], const float hi[3]) {
     for (int d = 0; d < 3; ++d) {
-        if (!e->ext_any) {
-            e->ext_lo[d] = lo[d];
-            e->ext_hi[d] = hi[d];
+        if (!sc.ext_any) {
+            sc.ext_lo[d] = lo[d];
+            sc.ext_hi[d] = hi[d];
         } else {
-            e->ext_lo[d] = std::min(e->ext_lo[d], lo[d]);
-            e->ext_hi[d] = std::max(e->ext_hi[d], hi[d]);
+            sc.ext_lo[d] = std::min(sc.ext_lo[d], lo[d]);
+            sc.ext_hi[d] = std::max(sc.ext_hi[d], hi[d]);
         }
     }
-    e->ext_any = true;
+    sc.ext_any = true;
 }
 
-int bounds_box(const sphk_params& p, float lo[3], float hi[3]) {
-    const float a[3] = {p.xmin, p.ymin, p.zmin};
-    const float b[3] = {p.xmax, p.ymax, p.zmax};
+// Box of the wall bounds (finite positions lie inside it after every step's clamp).
+int bounds_box(const float b[6], float lo[3], float hi[3]) {
     for (int d = 0; d < 3; ++d) {
-        if (!std::isfinite(a[d]) || !std::isfinite(b[d])) return -1;
-        lo[d] = std::min(a[d], b[d]);
-        hi[d] = std::max(a[d], b[d]);
+        const float a = b[2 * d], c = b[2 * d + 1];
+        if (!std::isfinite(a) || !std::isfinite(c)) return -1;
+        lo[d] = std::min(a, c);
+        hi[d] = std::max(a, c);
     }
     return 0;
 }
 
-// Dense grid covering every cell a finite position in [ext_lo, ext_hi] can fall in,
-// padded by two cells per side so the +-2 walk never leaves the grid.
-int grid_for_ext(const sphk_engine* e, Grid* g) {
-    const float inv = 1.0f / e->params.neighbor_search_radius;
+void params_bounds(const sphk_params& p, float b[6]) {
+    b[0] = p.xmin;
+    b[1] = p.xmax;
+    b[2] = p.ymin;
+    b[3] = p.ymax;
+    b[4] = p.zmin;
+    b[5] = p.zmax;
+}
+
+// Dense grid covering every cell a finite position in the scene's extent can fall in,
+// padded by two cells per side so the +-2 walk never leaves it.
+int grid_for_ext(const SceneHost& sc, float radius, Grid* g) {
+    const float inv = 1.0f / radius;
     int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    if (e->ext_any) {
-        const float* l = e->ext_lo;
-        const float* h = e->ext_hi;
+    if (sc.ext_any) {
         for (int d = 0; d < 3; ++d) {
-            if (floor_cell(l[d], inv, &lo[d]) || floor_cell(h[d], inv, &hi[d]))
+            if (floor_cell(sc.ext_lo[d], inv, &lo[d]) || floor_cell(sc.ext_hi[d], inv, &hi[d]))
                 return set_error(SPHK_E_DOMAIN,
                                  "particle coordinates need cell indices beyond +-2^30");
         }
@@ -232,7 +255,7 @@ int grid_for_ext(const sphk_engine* e, Grid* g) {
             return set_error(SPHK_E_DOMAIN,
                              "positions span more than %llu cells of size %g; the dense cell "
                              "grid supports at most that many",
-                             (unsigned long long)kMaxCells, (double)e->params.neighbor_search_radius);
+                             (unsigned long long)kMaxCells, (double)radius);
     }
     g->ox = (int32_t)(lo[0] - 2);
     g->oy = (int32_t)(lo[1] - 2);
@@ -244,13 +267,14 @@ int grid_for_ext(const sphk_engine* e, Grid* g) {
     return SPHK_OK;
 }
 
-__global__ void k_bbox(const float4* __restrict__ pm, uint32_t n, float* __restrict__ out6,
-                       uint32_t* __restrict__ any) {
-    // out6 = {min x,y,z as ordered-uint, max x,y,z} over finite positions
+__global__ void k_bbox(const float4* __restrict__ pm, uint32_t begin, uint32_t end,
+                       uint32_t* __restrict__ out7) {
+    // out7 = {min x,y,z, max x,y,z as order-preserving uints, any}
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     bool found = false;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x; i < end;
+         i += gridDim.x * blockDim.x) {
         float4 p = pm[i];
         if (__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z)) {
             lo[0] = fminf(lo[0], p.x);
@@ -262,17 +286,16 @@ __global__ void k_bbox(const float4* __restrict__ pm, uint32_t n, float* __restr
             found = true;
         }
     }
-    unsigned* u = reinterpret_cast<unsigned*>(out6);
     auto ord = [](float f) {
         unsigned b = __float_as_uint(f);
         return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
     };
     if (found) {
         for (int d = 0; d < 3; ++d) {
-            atomicMin(&u[d], ord(lo[d]));
-            atomicMax(&u[3 + d], ord(hi[d]));
+            atomicMin(&out7[d], ord(lo[d]));
+            atomicMax(&out7[3 + d], ord(hi[d]));
         }
-        atomicOr(any, 1u);
+        atomicOr(&out7[6], 1u);
     }
 }
 
@@ -283,78 +306,138 @@ float unord(unsigned b) {
     return f;
 }
 
-// Device bounding box of the finite positions (synchronous; used only when the
-// wall bounds are too wide to bound the grid).
+// Device bounding box of each scene's finite positions (synchronous; used after a restore
+// or a scene change, or when the wall bounds are too wide to bound the grid).
 int device_bbox(sphk_engine* e) {
+    const size_t S = e->scenes.size();
     uint32_t* buf = nullptr;
-    HIP_TRY(hipMalloc(&buf, 8 * sizeof(uint32_t)));
-    uint32_t init[8] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u, 0u, 0u};
-    HIP_TRY(hipMemcpyAsync(buf, init, sizeof(init), hipMemcpyHostToDevice, e->stream));
-    if (e->n) {
-        uint32_t blocks = (uint32_t)std::min<size_t>((e->n + 255) / 256, 1024);
-        hipLaunchKernelGGL(k_bbox, dim3(blocks), dim3(256), 0, e->stream, e->pm, (uint32_t)e->n,
-                           reinterpret_cast<float*>(buf), buf + 6);
-        HIP_TRY(hipGetLastError());
+    HIP_TRY(dalloc(&buf, 8 * S));
+    std::vector<uint32_t> init(8 * S, 0u);
+    for (size_t k = 0; k < S; ++k)
+        for (int d = 0; d < 3; ++d) init[8 * k + d] = 0xffffffffu;
+    HIP_TRY(hipMemcpyAsync(buf, init.data(), init.size() * 4, hipMemcpyHostToDevice, e->stream));
+    for (size_t k = 0; k < S; ++k) {
+        const uint64_t b = e->scenes[k].begin, en = std::min<uint64_t>(e->scenes[k].end, e->n);
+        if (en > b) {
+            uint32_t blocks = (uint32_t)std::min<uint64_t>((en - b + 255) / 256, 1024);
+            hipLaunchKernelGGL(k_bbox, dim3(blocks), dim3(256), 0, e->stream, e->pm, (uint32_t)b,
+                               (uint32_t)en, buf + 8 * k);
+            HIP_TRY(hipGetLastError());
+        }
     }
-    uint32_t out[8];
-    HIP_TRY(hipMemcpyAsync(out, buf, sizeof(out), hipMemcpyDeviceToHost, e->stream));
+    std::vector<uint32_t> out(8 * S);
+    HIP_TRY(hipMemcpyAsync(out.data(), buf, out.size() * 4, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     HIP_TRY(hipFree(buf));
-    e->ext_any = out[6] != 0;
-    for (int d = 0; d < 3; ++d) {
-        e->ext_lo[d] = e->ext_any ? unord(out[d]) : 0.0f;
-        e->ext_hi[d] = e->ext_any ? unord(out[3 + d]) : 0.0f;
+    for (size_t k = 0; k < S; ++k) {
+        SceneHost& sc = e->scenes[k];
+        sc.ext_any = out[8 * k + 6] != 0;
+        for (int d = 0; d < 3; ++d) {
+            sc.ext_lo[d] = sc.ext_any ? unord(out[8 * k + d]) : 0.0f;
+            sc.ext_hi[d] = sc.ext_any ? unord(out[8 * k + 3 + d]) : 0.0f;
+        }
     }
     e->ext_unknown = false;
     return SPHK_OK;
 }
 
+template <class T>
+int grow(T** p, uint64_t& cap, uint64_t want, uint64_t& gen) {
+    if (want <= cap) return SPHK_OK;
+    if (*p) HIP_TRY(hipFree(*p));
+    *p = nullptr;
+    uint64_t c = std::max(want, cap * 5 / 4);
+    HIP_TRY(dalloc(p, (size_t)c));
+    cap = c;
+    ++gen;
+    return SPHK_OK;
+}
+
+// Scene ranges: one implicit scene [0, n) with the parameter bounds, or the explicit ones.
+int check_scenes(sphk_engine* e) {
+    if (!e->scenes_explicit) {
+        if (e->scenes.size() != 1) e->scenes.resize(1);
+        e->scenes[0].begin = 0;
+        e->scenes[0].end = e->n;
+        params_bounds(e->params, e->scenes[0].bounds);
+        return SPHK_OK;
+    }
+    if (e->scenes.empty() || e->scenes.back().end != e->n)
+        return set_error(SPHK_E_INVALID, "scene ranges cover %llu particles, the engine holds %zu",
+                         (unsigned long long)(e->scenes.empty() ? 0 : e->scenes.back().end), e->n);
+    return SPHK_OK;
+}
+
 int ensure_grid(sphk_engine* e) {
+    if (int rc = check_scenes(e)) return rc;
     if (e->ext_unknown) {
-        int rc = device_bbox(e);
-        if (rc) return rc;
+        if (int rc = device_bbox(e)) return rc;
     }
-    Grid g;
-    int rc = grid_for_ext(e, &g);
-    if (rc) return rc;
-    if (g.G + 2 > e->cell_start_cap) {
-        if (e->cell_start) HIP_TRY(hipFree(e->cell_start));
-        uint64_t want = std::max<uint64_t>(g.G + 2, e->cell_start_cap * 5 / 4);
-        want = std::min<uint64_t>(want, kMaxCells + 2);
-        HIP_TRY(dalloc(&e->cell_start, (size_t)want));
-        e->cell_start_cap = want;
+    const size_t S = e->scenes.size();
+    uint64_t G = 0, ncols = 0;
+    e->table.resize(S);
+    for (size_t k = 0; k < S; ++k) {
+        SceneHost& sc = e->scenes[k];
+        if (int rc = grid_for_ext(sc, e->params.neighbor_search_radius, &sc.grid)) return rc;
+        SceneGrid& t = e->table[k];
+        t.ox = sc.grid.ox;
+        t.oy = sc.grid.oy;
+        t.oz = sc.grid.oz;
+        t.nx = sc.grid.nx;
+        t.ny = sc.grid.ny;
+        t.nz = sc.grid.nz;
+        t.cell_base = (uint32_t)G;
+        t.col_base = (uint32_t)ncols;
+        t.pbegin = (uint32_t)sc.begin;
+        t.pend = (uint32_t)sc.end;
+        for (int d = 0; d < 3; ++d) {
+            t.lo[d] = sc.bounds[2 * d];
+            t.hi[d] = sc.bounds[2 * d + 1];
+        }
+        G += sc.grid.G;
+        ncols += (uint64_t)sc.grid.nx * (uint64_t)sc.grid.ny;
+        if (G > kMaxCells)
+            return set_error(SPHK_E_DOMAIN, "scene grids need %llu cells; at most %llu supported",
+                             (unsigned long long)G, (unsigned long long)kMaxCells);
     }
-    e->grid = g;
-    e->grid_valid = true;
-    e->consts = make_consts(e->params, g);
+    if (int rc = grow(&e->cell_start, e->cell_start_cap, G + 2, e->buffer_gen)) return rc;
+    uint64_t scap = e->d_scenes_cap;
+    if (int rc = grow(&e->d_scenes, scap, S, e->buffer_gen)) return rc;
+    e->d_scenes_cap = scap;
+    HIP_TRY(hipMemcpyAsync(e->d_scenes, e->table.data(), S * sizeof(SceneGrid),
+                           hipMemcpyHostToDevice, e->stream));
+    e->consts = make_consts(e->params);
+    e->consts.G = (uint32_t)G;
+    e->consts.ncols = (uint32_t)ncols;
+    e->consts.nscenes = (uint32_t)S;
     // chunk work lists and neighbour words: at most ceil(C/64) + one partial chunk per column
-    const size_t ncols = (size_t)g.nx * (size_t)g.ny;
-    const size_t chunks = (e->cap + 63) / 64 + ncols;
+    const uint64_t chunks = (e->cap + 63) / 64 + ncols;
     WalkBuffers& wb = e->walk;
     if (chunks > wb.chunk_cap) {
         for (uint32_t* p : {wb.items, wb.mask, wb.wj0, wb.nwords})
             if (p) HIP_TRY(hipFree(p));
-        size_t want = std::max(chunks, wb.chunk_cap * 5 / 4);
+        size_t want = std::max<size_t>(chunks, wb.chunk_cap * 5 / 4);
         HIP_TRY(dalloc(&wb.items, want));
         HIP_TRY(dalloc(&wb.mask, want * kMaxWords * 64));
         HIP_TRY(dalloc(&wb.wj0, want * kMaxWords));
         HIP_TRY(dalloc(&wb.nwords, want));
         wb.chunk_cap = want;
+        ++e->buffer_gen;
     }
-    if (ncols + 1 > wb.col_cap) {
-        if (wb.colcnt) HIP_TRY(hipFree(wb.colcnt));
-        size_t want = std::max(ncols + 1, wb.col_cap * 5 / 4);
-        HIP_TRY(dalloc(&wb.colcnt, want));
-        wb.col_cap = want;
+    uint64_t ccap = wb.col_cap;
+    if (int rc = grow(&wb.colcnt, ccap, ncols + 1, e->buffer_gen)) return rc;
+    wb.col_cap = ccap;
+    if (!wb.nitems) {
+        HIP_TRY(dalloc(&wb.nitems, 1));
+        ++e->buffer_gen;
     }
-    if (!wb.nitems) HIP_TRY(dalloc(&wb.nitems, 1));
     const size_t hist_words = ((size_t)1 << kRadixMaxBits) * ((e->cap + kRadixTile - 1) / kRadixTile);
-    const size_t need = scan_scratch_words(std::max(hist_words, ncols + 1));
-    if (need > e->scan_scratch_cap) {
-        if (e->scan_scratch) HIP_TRY(hipFree(e->scan_scratch));
-        HIP_TRY(dalloc(&e->scan_scratch, need));
-        e->scan_scratch_cap = need;
-    }
+    uint64_t sscap = e->scan_scratch_cap;
+    if (int rc = grow(&e->scan_scratch, sscap,
+                      scan_scratch_words(std::max<size_t>(hist_words, ncols + 1)), e->buffer_gen))
+        return rc;
+    e->scan_scratch_cap = sscap;
+    e->grid_valid = true;
     return SPHK_OK;
 }
 
@@ -368,16 +451,16 @@ int total_key_bits(uint64_t G) {
 int build_sorted(sphk_engine* e) {
     const uint32_t n = (uint32_t)e->n;
     hipStream_t st = e->stream;
-    HIP_TRY(launch_cell_keys(e->pm, n, e->consts, e->keys, e->uscal, st));
+    HIP_TRY(launch_cell_keys(e->pm, n, e->consts, e->d_scenes, e->keys, e->uscal, st));
     uint32_t* ko = nullptr;
     uint32_t* vo = nullptr;
     HIP_TRY(radix_sort_keys(e->keys, e->keys_alt, e->vals, e->vals_alt, e->hist, e->scan_scratch,
-                            n, total_key_bits(e->grid.G), &ko, &vo, st));
+                            n, total_key_bits(e->consts.G), &ko, &vo, st));
     e->skey = ko;
     e->sid = vo;
-    HIP_TRY(launch_cell_start(e->skey, n, (uint32_t)e->grid.G, e->cell_start, st));
+    HIP_TRY(launch_cell_start(e->skey, n, e->consts.G, e->cell_start, st));
     HIP_TRY(launch_gather(e->pm, e->vel, e->sid, n, e->spm, e->svr, st));
-    HIP_TRY(launch_chunks(e->cell_start, e->consts, e->walk.colcnt, e->scan_scratch,
+    HIP_TRY(launch_chunks(e->cell_start, e->consts, e->d_scenes, e->walk.colcnt, e->scan_scratch,
                           e->walk.items, e->walk.nitems, st));
     e->sorted_valid = true;
     return SPHK_OK;
@@ -445,6 +528,7 @@ int sphk_create(size_t capacity, sphk_engine** out) {
     sphk_engine* e = new sphk_engine();
     e->cap = capacity;
     fill_default(&e->params);
+    e->scenes.resize(1);
     HIP_TRY(hipGetDevice(&e->device));
     HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     const size_t C = capacity;
@@ -490,8 +574,9 @@ int sphk_destroy(sphk_engine* e) {
         if (p) (void)hipFree(p);
     for (void* p : {(void*)e->walk.items, (void*)e->walk.nitems, (void*)e->walk.colcnt,
                     (void*)e->walk.mask, (void*)e->walk.wj0, (void*)e->walk.nwords,
-                    (void*)e->scan_scratch})
+                    (void*)e->scan_scratch, (void*)e->d_scenes})
         if (p) (void)hipFree(p);
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     for (auto* v : {&e->pending, &e->free_sets})
         for (auto& set : *v)
             for (auto& ev : set.ev) (void)hipEventDestroy(ev);
@@ -509,7 +594,12 @@ int sphk_set_params(sphk_engine* e, const sphk_params* p) {
         return set_error(SPHK_E_INVALID, "smoothing_length must be positive and finite");
     if (p->formulation != 0 && p->formulation != 1)
         return set_error(SPHK_E_INVALID, "formulation must be 0 or 1");
+    if (p->neighbor_search_radius != e->params.neighbor_search_radius) e->ext_unknown = true;
     e->params = *p;
+    if (!e->scenes_explicit) {
+        e->scenes.resize(1);
+        params_bounds(e->params, e->scenes[0].bounds);
+    }
     e->grid_valid = false;
     e->sorted_valid = false;
     return SPHK_OK;
@@ -530,8 +620,6 @@ int sphk_upload(sphk_engine* e, size_t offset, size_t n, const float* pos, const
     if (int rc = set_device(e)) return rc;
     size_t fit = std::min(n, e->cap - offset);
     std::vector<float4> hp(fit), hv(fit);
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    bool any = false;
     const float inv = 1.0f / e->params.neighbor_search_radius;
     for (size_t i = 0; i < fit; ++i) {
         float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
@@ -545,13 +633,6 @@ int sphk_upload(sphk_engine* e, size_t offset, size_t n, const float* pos, const
                 return set_error(SPHK_E_DOMAIN,
                                  "particle %zu at (%g, %g, %g) needs a cell index beyond +-2^30",
                                  offset + i, (double)x, (double)y, (double)z);
-            lo[0] = std::min(lo[0], x);
-            lo[1] = std::min(lo[1], y);
-            lo[2] = std::min(lo[2], z);
-            hi[0] = std::max(hi[0], x);
-            hi[1] = std::max(hi[1], y);
-            hi[2] = std::max(hi[2], z);
-            any = true;
         }
     }
     if (fit) {
@@ -561,17 +642,85 @@ int sphk_upload(sphk_engine* e, size_t offset, size_t n, const float* pos, const
                                hipMemcpyHostToDevice, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
     }
-    if (offset == 0) {
-        e->ext_any = false;
-        e->ext_unknown = false;
-    }
-    if (any) ext_add_box(e, lo, hi);
     e->n = offset + fit;
+    if (!e->scenes_explicit) {
+        // one implicit scene [0, n): grow its extent by the uploaded positions
+        e->scenes.resize(1);
+        SceneHost& sc = e->scenes[0];
+        if (offset == 0) sc.ext_any = false;
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        bool any = false;
+        for (size_t i = 0; i < fit; ++i) {
+            const float4& q = hp[i];
+            if (std::isfinite(q.x) && std::isfinite(q.y) && std::isfinite(q.z)) {
+                lo[0] = std::min(lo[0], q.x);
+                lo[1] = std::min(lo[1], q.y);
+                lo[2] = std::min(lo[2], q.z);
+                hi[0] = std::max(hi[0], q.x);
+                hi[1] = std::max(hi[1], q.y);
+                hi[2] = std::max(hi[2], q.z);
+                any = true;
+            }
+        }
+        if (any) ext_add_box(sc, lo, hi);
+        if (offset == 0) e->ext_unknown = false;
+    } else {
+        // explicit scenes: measure every scene's extent on the device before the next build
+        e->ext_unknown = true;
+    }
     e->grid_valid = false;
     e->sorted_valid = false;
     if (fit < n)
         return set_error(SPHK_E_CAPACITY, "particle capacity %zu exceeded: %zu particles dropped",
                          e->cap, n - fit);
+    return SPHK_OK;
+}
+
+int sphk_set_scenes(sphk_engine* e, size_t nscenes, const uint64_t* offsets, const float* bounds) {
+    if (int rc = check_engine(e)) return rc;
+    if (nscenes == 0) {
+        e->scenes_explicit = false;
+        e->scenes.assign(1, SceneHost{});
+        e->ext_unknown = true;
+        e->grid_valid = false;
+        e->sorted_valid = false;
+        return SPHK_OK;
+    }
+    if (!offsets) return set_error(SPHK_E_INVALID, "null scene offsets");
+    if (offsets[0] != 0) return set_error(SPHK_E_INVALID, "scene offsets must start at 0");
+    for (size_t k = 0; k < nscenes; ++k)
+        if (offsets[k + 1] < offsets[k])
+            return set_error(SPHK_E_INVALID, "scene offsets must not decrease");
+    if (offsets[nscenes] > e->cap)
+        return set_error(SPHK_E_CAPACITY, "scenes hold %llu particles, capacity %zu",
+                         (unsigned long long)offsets[nscenes], e->cap);
+    std::vector<SceneHost> sc(nscenes);
+    for (size_t k = 0; k < nscenes; ++k) {
+        sc[k].begin = offsets[k];
+        sc[k].end = offsets[k + 1];
+        if (bounds)
+            for (int d = 0; d < 6; ++d) sc[k].bounds[d] = bounds[6 * k + d];
+        else
+            params_bounds(e->params, sc[k].bounds);
+    }
+    e->scenes = std::move(sc);
+    e->scenes_explicit = true;
+    e->ext_unknown = true;
+    e->grid_valid = false;
+    e->sorted_valid = false;
+    return SPHK_OK;
+}
+
+int sphk_scene_count(const sphk_engine* e, size_t* out) {
+    if (int rc = check_engine(e)) return rc;
+    if (!out) return set_error(SPHK_E_INVALID, "null output");
+    *out = e->scenes.size();
+    return SPHK_OK;
+}
+
+int sphk_set_graph(sphk_engine* e, int enabled) {
+    if (int rc = check_engine(e)) return rc;
+    e->use_graph = enabled != 0;
     return SPHK_OK;
 }
 
@@ -587,7 +736,7 @@ int sphk_clear(sphk_engine* e) {
     if (int rc = set_device(e)) return rc;
     e->n = 0;
     e->step_count = 0;
-    e->ext_any = false;
+    for (auto& sc : e->scenes) sc.ext_any = false;
     e->ext_unknown = false;
     e->grid_valid = false;
     e->sorted_valid = false;
@@ -595,15 +744,72 @@ int sphk_clear(sphk_engine* e) {
     return SPHK_OK;
 }
 
+// Enqueues the kernels of one step on the engine's stream (eagerly or into a capture).
+int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev) {
+    hipStream_t st = e->stream;
+    const uint32_t n = (uint32_t)e->n;
+    if (ev) HIP_TRY(hipEventRecord(ev[0], st));
+    if (adaptive) HIP_TRY(launch_cfl(e->vel, e->acc, n, e->params, e->uscal + 1, e->fscal, st));
+    if (int rc = build_sorted(e)) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[1], st));
+    HIP_TRY(launch_density_walk(e->walk, e->spm, e->svr, e->skey, e->cell_start, n, e->consts,
+                                e->d_scenes, e->nstats, e->shape_density, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], st));
+    HIP_TRY(launch_force_walk(e->walk, e->spm, e->svr, e->skey, e->sid, e->cell_start, n,
+                              e->consts, e->d_scenes, adaptive ? e->fscal : nullptr, dt, e->pm,
+                              e->vel, e->acc, e->rho, e->prs, e->shape_force, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[3], st));
+    HIP_TRY(launch_advance_time(e->fscal + 1, adaptive ? e->fscal : nullptr, dt, st));
+    return SPHK_OK;
+}
+
+// Captures the step into a graph (or reuses the one captured for the same key) and
+// launches it.
+int graph_step(sphk_engine* e, float dt, bool adaptive) {
+    GraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.consts = e->consts;
+    key.n = (uint32_t)e->n;
+    key.shape_density = e->shape_density;
+    key.shape_force = e->shape_force;
+    key.adaptive = adaptive;
+    key.dt = adaptive ? 0.0f : dt;
+    key.buffer_gen = e->buffer_gen;
+    if (!e->graph_valid || memcmp(&key, &e->graph_key, sizeof(key)) != 0) {
+        if (e->graph_exec) {
+            HIP_TRY(hipGraphExecDestroy(e->graph_exec));
+            e->graph_exec = nullptr;
+        }
+        e->graph_valid = false;
+        HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(e, dt, adaptive, nullptr);
+        hipGraph_t graph = nullptr;
+        hipError_t ec = hipStreamEndCapture(e->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIP_TRY(ec);
+        hipError_t ei = hipGraphInstantiate(&e->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(ei);
+        memcpy(&e->graph_key, &key, sizeof(key));
+        e->graph_valid = true;
+    }
+    HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
+    return SPHK_OK;
+}
+
 int sphk_step(sphk_engine* e, float dt) {
     if (int rc = check_engine(e)) return rc;
     if (e->n == 0) return SPHK_OK;  // SPHEngine::step returns when empty (src/sph_engine.cpp:94)
     if (int rc = set_device(e)) return rc;
+    if (!e->grid_valid || e->ext_unknown || !e->scenes_explicit) {
+        if (int rc = check_scenes(e)) return rc;
+    }
     if (!e->grid_valid || e->ext_unknown) {
         if (int rc = ensure_grid(e)) return rc;
     }
-    hipStream_t st = e->stream;
-    const uint32_t n = (uint32_t)e->n;
     const bool adaptive = !(dt > 0.0f);
     hipEvent_t* ev = nullptr;
     // bound the events waiting to be read: fold the oldest in every 256 timed steps
@@ -619,45 +825,47 @@ int sphk_step(sphk_engine* e, float dt) {
         e->pending.push_back(e->free_sets.back());
         e->free_sets.pop_back();
         ev = e->pending.back().ev;
-        HIP_TRY(hipEventRecord(ev[0], st));
     }
-    if (adaptive) HIP_TRY(launch_cfl(e->vel, e->acc, n, e->params, e->uscal + 1, e->fscal, st));
-    if (int rc = build_sorted(e)) return rc;
-    if (ev) HIP_TRY(hipEventRecord(ev[1], st));
-    HIP_TRY(launch_density_walk(e->walk, e->spm, e->svr, e->skey, e->cell_start, n, e->consts,
-                                e->nstats, e->shape_density, st));
-    if (ev) HIP_TRY(hipEventRecord(ev[2], st));
-    HIP_TRY(launch_force_walk(e->walk, e->spm, e->svr, e->skey, e->sid, e->cell_start, n,
-                              e->consts, adaptive ? e->fscal : nullptr, dt, e->pm, e->vel, e->acc,
-                              e->rho, e->prs, e->shape_force, st));
-    if (ev) HIP_TRY(hipEventRecord(ev[3], st));
-    HIP_TRY(launch_advance_time(e->fscal + 1, adaptive ? e->fscal : nullptr, dt, st));
+    if (e->use_graph && !ev) {
+        if (int rc = graph_step(e, dt, adaptive)) return rc;
+    } else {
+        if (int rc = enqueue_step(e, dt, adaptive, ev)) return rc;
+    }
     e->sorted_valid = false;
     e->step_count++;
-    e->stats.total_neighbor_queries += n;
+    e->stats.total_neighbor_queries += e->n;
     e->stats.steps++;
-    // after the wall clamp every finite position lies in the bounds box
-    float lo[3], hi[3];
-    if (bounds_box(e->params, lo, hi) == 0) {
-        Grid g;
-        e->ext_any = true;
+    // after the wall clamp every finite position of a scene lies in its bounds box
+    bool changed = false;
+    for (auto& sc : e->scenes) {
+        float lo[3], hi[3];
+        if (bounds_box(sc.bounds, lo, hi) != 0) {
+            e->ext_unknown = true;
+            continue;
+        }
+        SceneHost next = sc;
+        next.ext_any = true;
         for (int d = 0; d < 3; ++d) {
-            e->ext_lo[d] = lo[d];
-            e->ext_hi[d] = hi[d];
+            next.ext_lo[d] = lo[d];
+            next.ext_hi[d] = hi[d];
         }
         std::string keep = g_last_error;
-        int rc = grid_for_ext(e, &g);
-        if (rc == SPHK_OK) {
-            if (g.G != e->grid.G || g.ox != e->grid.ox || g.oy != e->grid.oy || g.oz != e->grid.oz ||
-                g.nx != e->grid.nx || g.ny != e->grid.ny || g.nz != e->grid.nz)
-                e->grid_valid = false;
-        } else {
-            e->ext_unknown = true;  // walls too wide to bound the grid: measure the live bbox next step
-            g_last_error = keep;
+        Grid g;
+        if (grid_for_ext(next, e->params.neighbor_search_radius, &g) != SPHK_OK) {
+            g_last_error = keep;  // walls too wide to bound the grid: measure the live bbox
+            e->ext_unknown = true;
+            continue;
         }
-    } else {
-        e->ext_unknown = true;
+        if (g.G != sc.grid.G || g.ox != sc.grid.ox || g.oy != sc.grid.oy || g.oz != sc.grid.oz ||
+            g.nx != sc.grid.nx || g.ny != sc.grid.ny || g.nz != sc.grid.nz)
+            changed = true;
+        for (int d = 0; d < 3; ++d) {
+            sc.ext_lo[d] = lo[d];
+            sc.ext_hi[d] = hi[d];
+        }
+        sc.ext_any = true;
     }
+    if (changed) e->grid_valid = false;
     return SPHK_OK;
 }
 
@@ -830,11 +1038,7 @@ int sphk_snapshot_save(sphk_engine* e) {
     HIP_TRY(hipMemcpyAsync(e->fscal + 2, e->fscal + 1, sizeof(float), hipMemcpyDeviceToDevice, e->stream));
     e->snap_n = n;
     e->snap_steps = e->step_count;
-    for (int d = 0; d < 3; ++d) {
-        e->snap_ext_lo[d] = e->ext_lo[d];
-        e->snap_ext_hi[d] = e->ext_hi[d];
-    }
-    e->snap_ext_any = e->ext_any;
+    e->snap_scenes = e->scenes;
     e->snap_ext_unknown = e->ext_unknown;
     e->has_snapshot = true;
     return SPHK_OK;
@@ -854,12 +1058,20 @@ int sphk_snapshot_restore(sphk_engine* e) {
     e->n = n;
     e->step_count = e->snap_steps;
     e->sorted_valid = false;
-    for (int d = 0; d < 3; ++d) {
-        e->ext_lo[d] = e->snap_ext_lo[d];
-        e->ext_hi[d] = e->snap_ext_hi[d];
+    if (e->snap_scenes.size() == e->scenes.size()) {
+        for (size_t k = 0; k < e->scenes.size(); ++k) {
+            SceneHost& sc = e->scenes[k];
+            const SceneHost& sn = e->snap_scenes[k];
+            for (int d = 0; d < 3; ++d) {
+                sc.ext_lo[d] = sn.ext_lo[d];
+                sc.ext_hi[d] = sn.ext_hi[d];
+            }
+            sc.ext_any = sn.ext_any;
+        }
+        e->ext_unknown = e->snap_ext_unknown;
+    } else {
+        e->ext_unknown = true;
     }
-    e->ext_any = e->snap_ext_any;
-    e->ext_unknown = e->snap_ext_unknown;
     e->grid_valid = false;
     return SPHK_OK;
 }
@@ -886,14 +1098,17 @@ int sphk_debug_cell_coords(sphk_engine* e, int32_t* coords, size_t count) {
     std::vector<float> pos(n * 3);
     if (int rc = sphk_download(e, SPHK_POSITIONS, pos.data(), pos.size())) return rc;
     const float inv = 1.0f / e->params.neighbor_search_radius;
-    const Grid& g = e->grid;
     for (size_t s = 0; s < n; ++s) {
         uint32_t id = sid[s];
         uint32_t L = skey[s];
         int32_t* c = coords + 3 * (size_t)id;
-        if (L < g.G) {
-            uint32_t rz = L % (uint32_t)g.nz;
-            uint32_t t = L / (uint32_t)g.nz;
+        if (L < e->consts.G) {
+            size_t k = 0;
+            while (k + 1 < e->table.size() && e->table[k + 1].cell_base <= L) ++k;
+            const SceneGrid& g = e->table[k];
+            uint32_t local = L - g.cell_base;
+            uint32_t rz = local % (uint32_t)g.nz;
+            uint32_t t = local / (uint32_t)g.nz;
             uint32_t ry = t % (uint32_t)g.ny;
             uint32_t rx = t / (uint32_t)g.ny;
             c[0] = g.ox + (int32_t)rx;
@@ -926,6 +1141,7 @@ int sphk_debug_cell_keys(sphk_engine* e, uint64_t* keys, size_t count) {
 
 static int prepare_sorted(sphk_engine* e) {
     if (int rc = set_device(e)) return rc;
+    if (int rc = check_scenes(e)) return rc;
     if (!e->grid_valid || e->ext_unknown) {
         if (int rc = ensure_grid(e)) return rc;
     }
@@ -958,7 +1174,8 @@ int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t 
     HIP_TRY(dalloc(&dcnt, n));
     HIP_TRY(launch_fill_u32(dcnt, 0u, n, e->stream));
     HIP_TRY(launch_nbr_lists(e->walk, e->spm, e->skey, e->sid, e->cell_start, (uint32_t)n,
-                             e->consts, 0, dcnt, nullptr, nullptr, e->shape_density, e->stream));
+                             e->consts, e->d_scenes, 0, dcnt, nullptr, nullptr, e->shape_density,
+                             e->stream));
     std::vector<uint32_t> cnt(n);
     if (n) HIP_TRY(hipMemcpyAsync(cnt.data(), dcnt, n * 4, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -980,7 +1197,8 @@ int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t 
     HIP_TRY(dalloc(&didx, (size_t)o));
     HIP_TRY(hipMemcpyAsync(doffs, offs.data(), (n + 1) * 8, hipMemcpyHostToDevice, e->stream));
     HIP_TRY(launch_nbr_lists(e->walk, e->spm, e->skey, e->sid, e->cell_start, (uint32_t)n,
-                             e->consts, 1, nullptr, doffs, didx, e->shape_density, e->stream));
+                             e->consts, e->d_scenes, 1, nullptr, doffs, didx, e->shape_density,
+                             e->stream));
     if (o) HIP_TRY(hipMemcpyAsync(idx, didx, (size_t)o * 8, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     HIP_TRY(hipFree(doffs));
@@ -992,17 +1210,18 @@ int sphk_debug_grid(sphk_engine* e, int32_t* origin3, int32_t* dims3, uint64_t* 
     if (int rc = check_engine(e)) return rc;
     if (int rc = prepare_sorted(e)) return rc;
     if (int rc = check_grid_flag(e)) return rc;
+    const Grid& g = e->scenes[0].grid;
     if (origin3) {
-        origin3[0] = e->grid.ox;
-        origin3[1] = e->grid.oy;
-        origin3[2] = e->grid.oz;
+        origin3[0] = g.ox;
+        origin3[1] = g.oy;
+        origin3[2] = g.oz;
     }
     if (dims3) {
-        dims3[0] = e->grid.nx;
-        dims3[1] = e->grid.ny;
-        dims3[2] = e->grid.nz;
+        dims3[0] = g.nx;
+        dims3[1] = g.ny;
+        dims3[2] = g.nz;
     }
-    if (ncells) *ncells = e->grid.G;
+    if (ncells) *ncells = e->consts.G;
     return SPHK_OK;
 }
 

@@ -18,8 +18,9 @@
 
 namespace sphk {
 
-// Kernel-side constants of one step.  Derived on the host from sphk_params with
-// the reference's own FP32 expressions (src/kernels.cpp:23-36, src/spatial_hash.h:63-66).
+// Kernel-side constants of one step shared by every scene.  Derived on the host from
+// sphk_params with the reference's own FP32 expressions (src/kernels.cpp:23-36,
+// src/spatial_hash.h:63-66).
 struct StepConsts {
     float inv_cell;  // 1.0f / neighbor_search_radius
     float r2;        // neighbor_search_radius^2
@@ -29,11 +30,22 @@ struct StepConsts {
     float grad_scale;  // sigma / h          (applied once per particle)
     float lap_scale;   // sigma * mu / h^2   (applied once per particle)
     float k_gas, rho0, grav, damping;
-    float lo[3], hi[3];  // wall clamp bounds
     int32_t formulation;
-    int32_t ox, oy, oz;  // grid origin cell (absolute cell coordinates)
-    int32_t nx, ny, nz;  // grid dims (z fastest)
-    uint32_t G;          // nx*ny*nz; particles with dense index G have non-finite positions
+    uint32_t G;       // cells of all scene grids; particles with dense index G have a
+                      // non-finite coordinate
+    uint32_t ncols;   // (x, y) columns of all scene grids
+    uint32_t nscenes;
+};
+
+// Dense cell grid of one scene (z fastest) and its place in the global cell/column
+// numbering.  Scenes are independent: no neighbour search crosses two scene grids.
+struct SceneGrid {
+    int32_t ox, oy, oz;  // origin cell (absolute cell coordinates)
+    int32_t nx, ny, nz;  // dims, including two padding cells per side
+    uint32_t cell_base;  // first global cell index
+    uint32_t col_base;   // first global column index
+    uint32_t pbegin, pend;  // particle (caller id) range
+    float lo[3], hi[3];  // wall clamp bounds
 };
 
 struct Grid {
@@ -66,11 +78,42 @@ struct WalkBuffers {
     size_t col_cap = 0;
 };
 
+// Scene lookups by binary search over the scene table (few scenes; uniform per wave where
+// used by the walks).
+__device__ __forceinline__ uint32_t scene_of_cell(const SceneGrid* __restrict__ sc, uint32_t ns,
+                                                  uint32_t L) {
+    uint32_t lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (sc[mid].cell_base <= L) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t scene_of_col(const SceneGrid* __restrict__ sc, uint32_t ns,
+                                                 uint32_t col) {
+    uint32_t lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (sc[mid].col_base <= col) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t scene_of_particle(const SceneGrid* __restrict__ sc,
+                                                      uint32_t ns, uint32_t id) {
+    uint32_t lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (sc[mid].pbegin <= id) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 // launchers; all enqueue on `st`
 size_t scan_scratch_words(size_t m);
 hipError_t scan_exclusive(uint32_t* data, uint32_t m, uint32_t* scratch, hipStream_t st);
-hipError_t launch_cell_keys(const float4* pm, uint32_t n, const StepConsts& c, uint32_t* keys,
-                            uint32_t* err_flag, hipStream_t st);
+hipError_t launch_cell_keys(const float4* pm, uint32_t n, const StepConsts& c,
+                            const SceneGrid* scenes, uint32_t* keys, uint32_t* err_flag,
+                            hipStream_t st);
 hipError_t radix_sort_keys(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
                            uint32_t* hist, uint32_t* scan_scratch, uint32_t n, int total_bits,
                            uint32_t** keys_out, uint32_t** vals_out, hipStream_t st);
@@ -80,23 +123,24 @@ hipError_t launch_cell_start(const uint32_t* skey, uint32_t n, uint32_t G, uint3
 hipError_t launch_gather(const float4* pm, const float4* vel, const uint32_t* sid, uint32_t n,
                          float4* spm, float4* svr, hipStream_t st);
 uint32_t max_chunks(uint32_t n, const StepConsts& c);
-hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, uint32_t* cnt_scratch,
-                         uint32_t* scan_scratch, uint32_t* items, uint32_t* nitems,
-                         hipStream_t st);
+hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, const SceneGrid* scenes,
+                         uint32_t* cnt_scratch, uint32_t* scan_scratch, uint32_t* items,
+                         uint32_t* nitems, hipStream_t st);
 hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, float4* svr,
                                const uint32_t* skey, const uint32_t* cell_start, uint32_t n,
-                               const StepConsts& c, unsigned long long* nbr_stats, int shape,
-                               hipStream_t st);
+                               const StepConsts& c, const SceneGrid* scenes,
+                               unsigned long long* nbr_stats, int shape, hipStream_t st);
 hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const float4* svr,
                              const uint32_t* skey, const uint32_t* sid,
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
-                             const float* dt_dev, float dt, float4* pm, float4* vel, float4* acc,
-                             float* rho, float* prs, int shape, hipStream_t st);
+                             const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
+                             float4* vel, float4* acc, float* rho, float* prs, int shape,
+                             hipStream_t st);
 hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint32_t* skey,
                             const uint32_t* sid, const uint32_t* cell_start, uint32_t n,
-                            const StepConsts& c, int pass, uint32_t* count_by_id,
-                            const int64_t* offsets_by_id, int64_t* idx, int shape,
-                            hipStream_t st);
+                            const StepConsts& c, const SceneGrid* scenes, int pass,
+                            uint32_t* count_by_id, const int64_t* offsets_by_id, int64_t* idx,
+                            int shape, hipStream_t st);
 hipError_t launch_cfl(const float4* vel, const float4* acc, uint32_t n, const sphk_params& p,
                       uint32_t* scratch_u32, float* dt_dev, hipStream_t st);
 hipError_t launch_advance_time(float* time_dev, const float* dt_dev, float dt, hipStream_t st);

@@ -22,23 +22,27 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // K1: dense cell index per particle (caller order)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_cell_keys(const float4* __restrict__ pm, uint32_t n,
-                                                   StepConsts c, uint32_t* __restrict__ keys,
+                                                   StepConsts c,
+                                                   const SceneGrid* __restrict__ scenes,
+                                                   uint32_t* __restrict__ keys,
                                                    uint32_t* __restrict__ err) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 p = pm[i];
     uint32_t L = c.G;
     if (__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z)) {
+        const SceneGrid g = scenes[scene_of_particle(scenes, c.nscenes, i)];
         float fx = floorf(p.x * c.inv_cell);
         float fy = floorf(p.y * c.inv_cell);
         float fz = floorf(p.z * c.inv_cell);
         bool inr = fx >= -1073741824.0f && fx < 1073741824.0f && fy >= -1073741824.0f &&
                    fy < 1073741824.0f && fz >= -1073741824.0f && fz < 1073741824.0f;
-        int rx = inr ? (int)fx - c.ox : -1;
-        int ry = inr ? (int)fy - c.oy : -1;
-        int rz = inr ? (int)fz - c.oz : -1;
-        if (rx >= 2 && rx < c.nx - 2 && ry >= 2 && ry < c.ny - 2 && rz >= 2 && rz < c.nz - 2) {
-            L = ((uint32_t)rx * (uint32_t)c.ny + (uint32_t)ry) * (uint32_t)c.nz + (uint32_t)rz;
+        int rx = inr ? (int)fx - g.ox : -1;
+        int ry = inr ? (int)fy - g.oy : -1;
+        int rz = inr ? (int)fz - g.oz : -1;
+        if (rx >= 2 && rx < g.nx - 2 && ry >= 2 && ry < g.ny - 2 && rz >= 2 && rz < g.nz - 2) {
+            L = g.cell_base +
+                ((uint32_t)rx * (uint32_t)g.ny + (uint32_t)ry) * (uint32_t)g.nz + (uint32_t)rz;
         } else {
             atomicOr(err, 1u);
         }
@@ -46,11 +50,12 @@ __global__ __launch_bounds__(256) void k_cell_keys(const float4* __restrict__ pm
     keys[i] = L;
 }
 
-hipError_t launch_cell_keys(const float4* pm, uint32_t n, const StepConsts& c, uint32_t* keys,
-                            uint32_t* err_flag, hipStream_t st) {
+hipError_t launch_cell_keys(const float4* pm, uint32_t n, const StepConsts& c,
+                            const SceneGrid* scenes, uint32_t* keys, uint32_t* err_flag,
+                            hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_keys, dim3((n + 255) / 256), dim3(256), 0, st, pm, n, c, keys,
-                       err_flag);
+    hipLaunchKernelGGL(k_cell_keys, dim3((n + 255) / 256), dim3(256), 0, st, pm, n, c, scenes,
+                       keys, err_flag);
     return hipGetLastError();
 }
 

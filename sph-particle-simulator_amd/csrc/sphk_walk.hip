@@ -146,6 +146,9 @@ __device__ __forceinline__ float w_shape(float q) {
 // Wave-uniform description of one chunk and its candidate columns.
 struct Chunk {
     uint32_t s0, s_end;   // sorted slots [s0, s_end)
+    uint32_t scene;       // scene whose grid holds the chunk
+    uint32_t cell_base;   // that grid's first global cell
+    uint32_t ny, nz;      // that grid's dims (y, z)
     int rx, ry;           // column (grid-relative)
     int dx0, dx1, dy0, dy1;
     int zlo, zhi;         // candidate z cells (inclusive), grid-relative
@@ -161,8 +164,8 @@ struct Cursor {
 __device__ __forceinline__ void column_range(const Chunk& ch, int dx, int dy,
                                              const uint32_t* __restrict__ cs, const StepConsts& c,
                                              uint32_t& a, uint32_t& b) {
-    uint32_t base = ((uint32_t)(ch.rx + dx) * (uint32_t)c.ny + (uint32_t)(ch.ry + dy)) *
-                    (uint32_t)c.nz;
+    uint32_t base = ch.cell_base +
+                    ((uint32_t)(ch.rx + dx) * ch.ny + (uint32_t)(ch.ry + dy)) * ch.nz;
     a = uni(cs[base + (uint32_t)ch.zlo]);
     b = uni(cs[base + (uint32_t)ch.zhi + 1u]);
 }
@@ -262,18 +265,23 @@ __device__ __forceinline__ bool load_chunk(uint32_t item, const uint32_t* __rest
                                            const uint32_t* __restrict__ skey,
                                            const uint32_t* __restrict__ cs,
                                            const float4* __restrict__ spm, const StepConsts& c,
-                                           uint32_t lane, Chunk& ch, uint32_t& s, bool& active,
-                                           float4& pi) {
+                                           const SceneGrid* __restrict__ scenes, uint32_t lane,
+                                           Chunk& ch, uint32_t& s, bool& active, float4& pi) {
     if (item >= uni(nitems[0])) return false;
     ch.s0 = uni(items[item]);
     const uint32_t L0 = uni(skey[ch.s0]);
-    const uint32_t col = L0 / (uint32_t)c.nz;
-    const uint32_t col_end = uni(cs[(col + 1u) * (uint32_t)c.nz]);
+    ch.scene = uni(scene_of_cell(scenes, c.nscenes, L0));
+    ch.cell_base = uni(scenes[ch.scene].cell_base);
+    ch.ny = uni((uint32_t)scenes[ch.scene].ny);
+    ch.nz = uni((uint32_t)scenes[ch.scene].nz);
+    const uint32_t local = L0 - ch.cell_base;
+    const uint32_t col = local / ch.nz;
+    const uint32_t col_end = uni(cs[ch.cell_base + (col + 1u) * ch.nz]);
     ch.s_end = min(ch.s0 + 64u, col_end);
-    ch.rx = (int)(col / (uint32_t)c.ny);
-    ch.ry = (int)(col % (uint32_t)c.ny);
-    const int zmin = (int)(L0 % (uint32_t)c.nz);
-    const int zmax = (int)(uni(skey[ch.s_end - 1u]) % (uint32_t)c.nz);
+    ch.rx = (int)(col / ch.ny);
+    ch.ry = (int)(col % ch.ny);
+    const int zmin = (int)(local % ch.nz);
+    const int zmax = (int)((uni(skey[ch.s_end - 1u]) - ch.cell_base) % ch.nz);
     s = ch.s0 + lane;
     active = s < ch.s_end;
     const float qnan = __int_as_float(0x7fc00000);
@@ -355,46 +363,50 @@ __device__ __forceinline__ void store_words(uint32_t item, int seg_base, int nw,
 // ---------------------------------------------------------------------------
 // chunk list: per column, ceil(len/64) chunks of consecutive sorted slots
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_chunk_count(const uint32_t* __restrict__ cs, uint32_t ncols,
-                                                     uint32_t nz, uint32_t* __restrict__ cnt) {
-    uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col > ncols) return;
-    if (col == ncols) {
+__global__ __launch_bounds__(256) void k_chunk_count(const uint32_t* __restrict__ cs,
+                                                     StepConsts c,
+                                                     const SceneGrid* __restrict__ scenes,
+                                                     uint32_t* __restrict__ cnt) {
+    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col > c.ncols) return;
+    if (col == c.ncols) {
         cnt[col] = 0;
         return;
     }
-    uint32_t a = cs[col * nz], b = cs[(col + 1) * nz];
+    const SceneGrid g = scenes[scene_of_col(scenes, c.nscenes, col)];
+    const uint32_t first = g.cell_base + (col - g.col_base) * (uint32_t)g.nz;
+    const uint32_t a = cs[first], b = cs[first + (uint32_t)g.nz];
     cnt[col] = (b - a + 63u) / 64u;
 }
 
-__global__ __launch_bounds__(256) void k_chunk_fill(const uint32_t* __restrict__ cs, uint32_t ncols,
-                                                    uint32_t nz, const uint32_t* __restrict__ offs,
+__global__ __launch_bounds__(256) void k_chunk_fill(const uint32_t* __restrict__ cs, StepConsts c,
+                                                    const SceneGrid* __restrict__ scenes,
+                                                    const uint32_t* __restrict__ offs,
                                                     uint32_t* __restrict__ items,
                                                     uint32_t* __restrict__ nitems) {
-    uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col == 0) nitems[0] = offs[ncols];
-    if (col >= ncols) return;
-    uint32_t a = cs[col * nz], b = cs[(col + 1) * nz];
+    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col == 0) nitems[0] = offs[c.ncols];
+    if (col >= c.ncols) return;
+    const SceneGrid g = scenes[scene_of_col(scenes, c.nscenes, col)];
+    const uint32_t first = g.cell_base + (col - g.col_base) * (uint32_t)g.nz;
+    const uint32_t a = cs[first], b = cs[first + (uint32_t)g.nz];
     uint32_t o = offs[col];
     for (uint32_t s = a; s < b; s += 64u) items[o++] = s;
 }
 
-hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, uint32_t* cnt_scratch,
-                         uint32_t* scan_scratch, uint32_t* items, uint32_t* nitems,
-                         hipStream_t st) {
-    const uint32_t ncols = (uint32_t)c.nx * (uint32_t)c.ny;
-    hipLaunchKernelGGL(k_chunk_count, dim3((ncols + 1 + 255) / 256), dim3(256), 0, st, cell_start,
-                       ncols, (uint32_t)c.nz, cnt_scratch);
-    hipError_t e = scan_exclusive(cnt_scratch, ncols + 1, scan_scratch, st);
+hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, const SceneGrid* scenes,
+                         uint32_t* cnt_scratch, uint32_t* scan_scratch, uint32_t* items,
+                         uint32_t* nitems, hipStream_t st) {
+    hipLaunchKernelGGL(k_chunk_count, dim3((c.ncols + 1 + 255) / 256), dim3(256), 0, st,
+                       cell_start, c, scenes, cnt_scratch);
+    hipError_t e = scan_exclusive(cnt_scratch, c.ncols + 1, scan_scratch, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_chunk_fill, dim3((ncols + 255) / 256 + 1), dim3(256), 0, st, cell_start,
-                       ncols, (uint32_t)c.nz, cnt_scratch, items, nitems);
+    hipLaunchKernelGGL(k_chunk_fill, dim3((c.ncols + 255) / 256 + 1), dim3(256), 0, st, cell_start,
+                       c, scenes, cnt_scratch, items, nitems);
     return hipGetLastError();
 }
 
-uint32_t max_chunks(uint32_t n, const StepConsts& c) {
-    return (n + 63u) / 64u + (uint32_t)c.nx * (uint32_t)c.ny;
-}
+uint32_t max_chunks(uint32_t n, const StepConsts& c) { return (n + 63u) / 64u + c.ncols; }
 
 // ---------------------------------------------------------------------------
 // Launch shapes of the walks: words per LDS segment x wavefronts per workgroup.  Every
@@ -415,8 +427,8 @@ template <class Shape>
 __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, float4* __restrict__ svr, const uint32_t* __restrict__ skey,
-    const uint32_t* __restrict__ cs, StepConsts c, uint32_t* __restrict__ gmask,
-    uint32_t* __restrict__ gwj0, uint32_t* __restrict__ gnw,
+    const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
+    uint32_t* __restrict__ gmask, uint32_t* __restrict__ gwj0, uint32_t* __restrict__ gnw,
     unsigned long long* __restrict__ stats) {
     constexpr int kSeg = Shape::seg;
     __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
@@ -430,7 +442,7 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     uint32_t s;
     bool active;
     float4 pi;
-    if (!load_chunk(item, items, nitems, skey, cs, spm, c, lane, ch, s, active, pi)) return;
+    if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
     Cursor cu;
     cursor_begin(cu, ch, cs, c);
     float acc = 0.0f;
@@ -471,7 +483,8 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
 // apply_boundary_conditions (src/particle.cpp:122-153).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void integrate_clamp(float4& p, float3& v, float3 a, float dt,
-                                                const StepConsts& c) {
+                                                const StepConsts& c, const float* lo,
+                                                const float* hi) {
 #pragma clang fp contract(off)
     float hx = 0.5f * a.x, hy = 0.5f * a.y, hz = 0.5f * a.z;
     v.x = v.x + hx * dt;
@@ -487,25 +500,25 @@ __device__ __forceinline__ void integrate_clamp(float4& p, float3& v, float3 a, 
     v.y = v.y * c.damping;
     v.z = v.z * c.damping;
     const float wd = -0.8f;
-    if (p.x < c.lo[0]) {
-        p.x = c.lo[0];
+    if (p.x < lo[0]) {
+        p.x = lo[0];
         v.x = v.x * wd;
-    } else if (p.x > c.hi[0]) {
-        p.x = c.hi[0];
+    } else if (p.x > hi[0]) {
+        p.x = hi[0];
         v.x = v.x * wd;
     }
-    if (p.y < c.lo[1]) {
-        p.y = c.lo[1];
+    if (p.y < lo[1]) {
+        p.y = lo[1];
         v.y = v.y * wd;
-    } else if (p.y > c.hi[1]) {
-        p.y = c.hi[1];
+    } else if (p.y > hi[1]) {
+        p.y = hi[1];
         v.y = v.y * wd;
     }
-    if (p.z < c.lo[2]) {
-        p.z = c.lo[2];
+    if (p.z < lo[2]) {
+        p.z = lo[2];
         v.z = v.z * wd;
-    } else if (p.z > c.hi[2]) {
-        p.z = c.hi[2];
+    } else if (p.z > hi[2]) {
+        p.z = hi[2];
         v.z = v.z * wd;
     }
 }
@@ -525,9 +538,10 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const float4* __restrict__ svr,
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sid,
-    const uint32_t* __restrict__ cs, StepConsts c, const uint32_t* __restrict__ gmask,
-    const uint32_t* __restrict__ gwj0, const uint32_t* __restrict__ gnw,
-    const float* __restrict__ dt_dev, float dt_arg, float4* __restrict__ pm,
+    const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
+    const uint32_t* __restrict__ gmask, const uint32_t* __restrict__ gwj0,
+    const uint32_t* __restrict__ gnw, const float* __restrict__ dt_dev, float dt_arg,
+    float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc_out, float* __restrict__ rho_out,
     float* __restrict__ prs_out) {
     constexpr int kSeg = Shape::seg;
@@ -542,7 +556,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     uint32_t s;
     bool active;
     float4 pi;
-    if (!load_chunk(item, items, nitems, skey, cs, spm, c, lane, ch, s, active, pi)) return;
+    if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
     const float4 vi = active ? svr[s] : make_float4(0.f, 0.f, 0.f, 1.f);
     const float rho_i = vi.w;
     const float P_i = c.k_gas * (rho_i - c.rho0);
@@ -621,7 +635,8 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     const float dt = dt_dev ? *dt_dev : dt_arg;
     float3 v = make_float3(vi.x, vi.y, vi.z);
     float4 p = pi;
-    integrate_clamp(p, v, a, dt, c);
+    const SceneGrid& g = scenes[ch.scene];
+    integrate_clamp(p, v, a, dt, c, g.lo, g.hi);
     const uint32_t id = sid[s];
     pm[id] = p;
     vel[id] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -634,7 +649,8 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
 // themselves: rho = m W(0), a = (0, g, 0); they are integrated and clamped like the rest.
 __global__ __launch_bounds__(256) void k_nonfinite_step(
     const float4* __restrict__ spm, float4* __restrict__ svr, const uint32_t* __restrict__ sid,
-    const uint32_t* __restrict__ cs, uint32_t n, StepConsts c, const float* __restrict__ dt_dev,
+    const uint32_t* __restrict__ cs, uint32_t n, StepConsts c,
+    const SceneGrid* __restrict__ scenes, const float* __restrict__ dt_dev,
     float dt_arg, float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc_out,
     float* __restrict__ rho_out, float* __restrict__ prs_out) {
     const uint32_t first = cs[c.G];
@@ -649,8 +665,9 @@ __global__ __launch_bounds__(256) void k_nonfinite_step(
     const float dt = dt_dev ? *dt_dev : dt_arg;
     float3 v = make_float3(vi.x, vi.y, vi.z);
     float4 p = pi;
-    integrate_clamp(p, v, a, dt, c);
     const uint32_t id = sid[s];
+    const SceneGrid& g = scenes[scene_of_particle(scenes, c.nscenes, id)];
+    integrate_clamp(p, v, a, dt, c, g.lo, g.hi);
     pm[id] = p;
     vel[id] = make_float4(v.x, v.y, v.z, 0.0f);
     acc_out[id] = make_float4(a.x, a.y, a.z, 0.0f);
@@ -666,7 +683,8 @@ template <class Shape>
 __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const uint32_t* __restrict__ skey,
-    const uint32_t* __restrict__ sid, const uint32_t* __restrict__ cs, StepConsts c, int pass,
+    const uint32_t* __restrict__ sid, const uint32_t* __restrict__ cs, StepConsts c,
+    const SceneGrid* __restrict__ scenes, int pass,
     uint32_t* __restrict__ cnt_by_id, const int64_t* __restrict__ offs_by_id,
     int64_t* __restrict__ idx) {
     constexpr int kSeg = Shape::seg;
@@ -680,8 +698,8 @@ __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
     uint32_t s;
     bool active;
     float4 pi;
-    if (!load_chunk(swizzled_block() * Shape::waves + wave, items, nitems, skey, cs, spm, c, lane, ch, s,
-                    active, pi))
+    if (!load_chunk(swizzled_block() * Shape::waves + wave, items, nitems, skey, cs, spm, c, scenes,
+                    lane, ch, s, active, pi))
         return;
     Cursor cu;
     cursor_begin(cu, ch, cs, c);
@@ -745,47 +763,48 @@ static uint32_t grid_for(uint32_t chunks) {
 
 hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, float4* svr,
                                const uint32_t* skey, const uint32_t* cell_start, uint32_t n,
-                               const StepConsts& c, unsigned long long* nbr_stats, int shape,
-                               hipStream_t st) {
+                               const StepConsts& c, const SceneGrid* scenes,
+                               unsigned long long* nbr_stats, int shape, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const uint32_t chunks = max_chunks(n, c);
     SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_density_walk<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, svr, skey, cell_start, c, wb.mask, wb.wj0,
-                                                  wb.nwords, nbr_stats));
+                                                  spm, svr, skey, cell_start, c, scenes, wb.mask,
+                                                  wb.wj0, wb.nwords, nbr_stats));
     return hipGetLastError();
 }
 
 hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const float4* svr,
                              const uint32_t* skey, const uint32_t* sid,
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
-                             const float* dt_dev, float dt, float4* pm, float4* vel, float4* acc,
-                             float* rho, float* prs, int shape, hipStream_t st) {
+                             const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
+                             float4* vel, float4* acc, float* rho, float* prs, int shape,
+                             hipStream_t st) {
     if (n == 0) return hipSuccess;
     const uint32_t chunks = max_chunks(n, c);
     SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_force_walk<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, svr, skey, sid, cell_start, c, wb.mask,
-                                                  wb.wj0, wb.nwords, dt_dev, dt, pm, vel, acc, rho,
-                                                  prs));
+                                                  spm, svr, skey, sid, cell_start, c, scenes,
+                                                  wb.mask, wb.wj0, wb.nwords, dt_dev, dt, pm, vel,
+                                                  acc, rho, prs));
     // the non-finite bin holds at most n particles; the kernel reads its start on the device
     hipLaunchKernelGGL(k_nonfinite_step, dim3((n + 255) / 256), dim3(256), 0, st, spm,
-                       const_cast<float4*>(svr), sid, cell_start, n, c, dt_dev, dt, pm, vel, acc,
-                       rho, prs);
+                       const_cast<float4*>(svr), sid, cell_start, n, c, scenes, dt_dev, dt, pm, vel,
+                       acc, rho, prs);
     return hipGetLastError();
 }
 
 hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint32_t* skey,
                             const uint32_t* sid, const uint32_t* cell_start, uint32_t n,
-                            const StepConsts& c, int pass, uint32_t* count_by_id,
-                            const int64_t* offsets_by_id, int64_t* idx, int shape,
-                            hipStream_t st) {
+                            const StepConsts& c, const SceneGrid* scenes, int pass,
+                            uint32_t* count_by_id, const int64_t* offsets_by_id, int64_t* idx,
+                            int shape, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const uint32_t chunks = max_chunks(n, c);
     SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_nbr_lists<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, skey, sid, cell_start, c, pass, count_by_id,
-                                                  offsets_by_id, idx));
+                                                  spm, skey, sid, cell_start, c, scenes, pass,
+                                                  count_by_id, offsets_by_id, idx));
     return hipGetLastError();
 }
 

@@ -92,6 +92,9 @@ SIGNATURES = {
     "sphk_debug_sorted": [_P, _U32, _U32, _SZ],
     "sphk_debug_neighbors": [_P, _I64, _I64, _SZ, C.POINTER(_SZ)],
     "sphk_debug_grid": [_P, _I32, _I32, _U64],
+    "sphk_set_scenes": [_P, _SZ, _U64, _F],
+    "sphk_scene_count": [_P, C.POINTER(_SZ)],
+    "sphk_set_graph": [_P, C.c_int],
     "sphk_set_walk_shapes": [_P, C.c_int, C.c_int],
     "sphk_get_walk_shapes": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "sphk_walk_shape_count": [],
@@ -267,6 +270,24 @@ class Engine:
 
     def snapshot_restore(self):
         check(self.lib.sphk_snapshot_restore(self.h))
+
+    def set_scenes(self, offsets, bounds=None):
+        """Independent scenes: particles [offsets[k], offsets[k+1]) with walls bounds[k]."""
+        if offsets is None or len(offsets) == 0:
+            check(self.lib.sphk_set_scenes(self.h, 0, None, None))
+            return
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        b = None if bounds is None else np.ascontiguousarray(bounds, dtype=np.float32).reshape(-1, 6)
+        check(self.lib.sphk_set_scenes(self.h, off.size - 1, _ptr(off, C.c_uint64),
+                                       _ptr(b, C.c_float)))
+
+    def scene_count(self) -> int:
+        v = _SZ()
+        check(self.lib.sphk_scene_count(self.h, C.byref(v)))
+        return v.value
+
+    def set_graph(self, on: bool):
+        check(self.lib.sphk_set_graph(self.h, int(bool(on))))
 
     def set_walk_shapes(self, density: int, force: int):
         check(self.lib.sphk_set_walk_shapes(self.h, int(density), int(force)))

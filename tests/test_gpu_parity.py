@@ -370,3 +370,89 @@ def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes):
                                     o.params)
     err = np.linalg.norm(e.accelerations().astype(np.float64) - o.get(4), axis=1) / scale
     assert err.max() <= ACC_TOL
+
+
+def test_graph_replay_matches_eager(sphk, scenes):
+    # a captured step replayed many times equals the eager step bit for bit, with eager work
+    # (downloads, restores) between the replays
+    s = scenes.config0(seed=5)
+    outs = []
+    for graph in (False, True):
+        e = _engine(sphk, s)
+        e.set_graph(graph)
+        e.snapshot_save()
+        res = []
+        for k in range(4):
+            e.snapshot_restore()
+            e.step(s.dt)
+            res.append((e.positions(), e.velocities(), e.densities()))
+        s2 = scenes.small_block(10, 10, 10, seed=8)
+        e2 = _engine(sphk, s2, formulation=1)
+        e2.set_graph(graph)
+        for k in range(5):
+            e2.step(1e-4)
+            res.append((e2.positions(), e2.densities()))
+        e2.step(0.0)  # adaptive dt: a new capture
+        res.append((e2.positions(),))
+        outs.append(res)
+    for a, b in zip(outs[0], outs[1]):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+
+
+def _ensemble(scenes_mod, n_scenes=6, scale=0.02, seed=3):
+    return scenes_mod.config3(seed=seed, n_scenes=n_scenes, scale=scale)
+
+
+def test_batched_scenes_equal_single_scene_runs(sphk, scenes):
+    # one engine stepping several independent scenes gives each scene exactly the results of
+    # an engine holding that scene alone
+    ens = _ensemble(scenes)
+    offs = np.cumsum([0] + [sc.n for sc in ens]).astype(np.uint64)
+    bounds = np.array([sc.bounds for sc in ens], np.float32)
+    pos = np.concatenate([sc.pos for sc in ens])
+    vel = np.concatenate([sc.vel for sc in ens])
+    eb = sphk.Engine(len(pos), ens[0].params(sphk))
+    eb.upload(pos, vel)
+    eb.set_scenes(offs, bounds)
+    assert eb.scene_count() == len(ens)
+    for k in range(3):
+        eb.step(0.001)
+    pb, vb, rb = eb.positions(), eb.velocities(), eb.densities()
+    for i, sc in enumerate(ens):
+        e = _engine(sphk, sc)
+        for k in range(3):
+            e.step(0.001)
+        a, b = int(offs[i]), int(offs[i + 1])
+        np.testing.assert_array_equal(pb[a:b], e.positions())
+        np.testing.assert_array_equal(vb[a:b], e.velocities())
+        np.testing.assert_array_equal(rb[a:b], e.densities()[: sc.n])
+
+
+def test_batched_scenes_against_oracle(sphk, oracle, scenes):
+    ens = _ensemble(scenes, n_scenes=4, scale=0.03, seed=9)
+    offs = np.cumsum([0] + [sc.n for sc in ens]).astype(np.uint64)
+    pos = np.concatenate([sc.pos for sc in ens])
+    eb = sphk.Engine(len(pos), ens[0].params(sphk))
+    eb.upload(pos)
+    eb.set_scenes(offs, np.array([sc.bounds for sc in ens], np.float32))
+    eb.set_graph(True)
+    eb.step(0.001)
+    rho = eb.densities()
+    acc = eb.accelerations()
+    for i, sc in enumerate(ens):
+        o = _oracle(oracle, sc)
+        _oracle_phases(o)
+        a, b = int(offs[i]), int(offs[i + 1])
+        np.testing.assert_allclose(rho[a:b], o.get(2), rtol=RHO_TOL)
+        offs_o, idx_o = o.neighbors()
+        mass = np.full(sc.n, sc.mass, np.float32)
+        scale = oracle.accel_term_scale(sc.pos, sc.vel, mass, o.get(2), offs_o, idx_o, o.params)
+        err = np.linalg.norm(acc[a:b].astype(np.float64) - o.get(4), axis=1) / scale
+        assert err.max() <= ACC_TOL
+    # scenes never see each other: neighbour lists stay inside each scene's id range
+    offs_n, idx_n = eb.neighbors()
+    for i in range(len(ens)):
+        a, b = int(offs[i]), int(offs[i + 1])
+        seg = idx_n[offs_n[a]:offs_n[b]]
+        assert seg.min() >= a and seg.max() < b
