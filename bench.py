@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default walk launch shapes instead of tuning them first")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-ensemble and graph-mode lines")
+    ap.add_argument("--ensemble-scale", type=float, default=1.0,
+                    help="particle-count scale of the 64-scene ensemble (1.0 = full size)")
     ap.add_argument("--continuous", type=int, default=0,
                     help="also time N continuous steps of the consistent formulation at dt=1e-4")
     return ap.parse_args()
@@ -73,6 +77,61 @@ def cpu_baseline(scene, repeats):
                   f"{med:.3f} s/step; C restatement of the reference step (hash-map build, "
                   f"125-cell query, per-particle neighbour vectors, OpenMP), reference flags",
     }
+
+
+def time_reset_steps(e, dt, steps, warmup):
+    """Reset-and-step timing of an engine whose initial state is its snapshot."""
+    for _ in range(warmup):
+        e.snapshot_restore()
+        e.step(dt)
+    e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.snapshot_restore()
+        e.step(dt)
+    e.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def ensemble_line(sphk, sph_scenes, scale, steps):
+    """64 independent dam-break scenes (log-uniform 10k..500k particles) stepped together
+    through the segmented cell grid, graph-captured, autotuned."""
+    import numpy as np
+
+    import sph_autotune
+
+    ens = sph_scenes.config3(seed=0, n_scenes=64, scale=scale)
+    offs = np.cumsum([0] + [sc.n for sc in ens]).astype(np.uint64)
+    pos = np.concatenate([sc.pos for sc in ens])
+    n = int(offs[-1])
+    e = sphk.Engine(n, ens[0].params(sphk))
+    e.upload(pos)
+    e.set_scenes(offs, np.array([sc.bounds for sc in ens], np.float32))
+    tune = sph_autotune.autotune(e, ens[0].dt, steps=2, use_cache=False)
+    e.snapshot_save()
+    e.set_graph(True)
+    t = time_reset_steps(e, ens[0].dt, steps, 2)
+    out = {"scenes": len(ens), "particles": n, "min_scene": int(min(sc.n for sc in ens)),
+           "max_scene": int(max(sc.n for sc in ens)), "ms_per_step": t * 1e3,
+           "value": n / t, "unit": "particle-updates/s", "graph": True,
+           "walk_shapes": [tune["names"][tune["density"]], tune["names"][tune["force"]]]}
+    del e
+    return out
+
+
+def graph_line(sphk, sph_scenes, steps):
+    """10,000-particle dam break, eager vs graph-captured step (launch-bound size)."""
+    s = sph_scenes.config0()
+    res = {}
+    for graph in (False, True):
+        e = sphk.Engine(s.n, s.params(sphk))
+        e.upload(s.pos, s.vel)
+        e.snapshot_save()
+        e.set_graph(graph)
+        t = time_reset_steps(e, s.dt, steps, 5)
+        res["graph" if graph else "eager"] = {"ms_per_step": t * 1e3, "value": s.n / t}
+        del e
+    return res
 
 
 def main():
@@ -182,6 +241,9 @@ def main():
                         "tuning_ms": None if tune is None else
                         {"density": tune["density_ms"], "force": tune["force_ms"]}},
     }
+    if not args.no_extras:
+        out["ensemble"] = ensemble_line(sphk, sph_scenes, args.ensemble_scale, 5)
+        out["small_scene_10k"] = graph_line(sphk, sph_scenes, 50)
     if args.continuous:
         e2 = sphk.Engine(scene.n, scene.params(sphk, formulation=1))
         e2.upload(scene.pos, scene.vel)
