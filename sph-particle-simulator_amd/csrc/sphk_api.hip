@@ -173,6 +173,8 @@ StepConsts make_consts(const sphk_params& p) {
     const float h = p.smoothing_length;
     c.inv_cell = 1.0f / p.neighbor_search_radius;  // SpatialHash::set_cell_size (src/spatial_hash.h:63-66)
     c.r2 = p.neighbor_search_radius * p.neighbor_search_radius;  // src/sph_engine.cpp:347
+    c.r2_filter = c.r2;
+    for (int k = 0; k < 8; ++k) c.r2_filter = std::nextafter(c.r2_filter, INFINITY);
     c.h = h;
     c.inv_h = 1.0f / h;
     if (p.formulation == 1)

@@ -60,6 +60,19 @@ __device__ __forceinline__ float dist2_exact(float ax, float ay, float az, float
     return (tx + ty) + tz;
 }
 
+// Contracted d^2 for the candidate filter: fl(dx^2 + fl(dy^2 + fl(dz^2))) with two fused
+// multiply-adds.  Both roundings stay within 3 units of 2^-24 of the true sum, so the two
+// values differ by under 7 units of 2^-24 relative, and
+// testing it against r2 raised by 8 ulp (StepConsts::r2_filter) keeps every pair the exact
+// test keeps; the few extra pairs are dropped by the exact test where the pair is used.
+__device__ __forceinline__ float dist2_filter(float ax, float ay, float az, float bx, float by,
+                                              float bz) {
+    const float dx = ax - bx;
+    const float dy = ay - by;
+    const float dz = az - bz;
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+
 // Can a particle at coordinate x (cell c = floor(fl(x*inv))) have a neighbour in cell c+2
 // (bit 1) or c-2 (bit 0)?  A neighbour needs fl(fl(b-x)^2) <= r2 on this axis alone, and
 // fl(b-x) grows with b, so it suffices to test the float b nearest to x that lands in cell
@@ -215,9 +228,9 @@ struct SegLane {
     uint32_t count;
 };
 
-// Phase 1: fills up to kSeg words of pass bits into LDS (masks[w*64 + lane], wj0[w]).
-// Candidate k of a word sits in bit 31-k (words of fewer than 32 candidates are
-// left-aligned).  Returns the number of words written.
+// Phase 1: fills up to kSeg words of filter bits into LDS (masks[w*64 + lane], wj0[w]): a
+// superset of the neighbours (see dist2_filter).  Candidate k of a word sits in bit 31-k
+// (words of fewer than 32 candidates are left-aligned).  Returns the number of words written.
 template <int kSeg>
 __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
                                               const float4* __restrict__ spm,
@@ -239,12 +252,12 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
 #pragma unroll
             for (int k = 0; k < 32; ++k) {
                 const float4 q = pb[k];
-                word = push_bit(word, dist2_exact(px, py, pz, q.x, q.y, q.z), c.r2);
+                word = push_bit(word, dist2_filter(px, py, pz, q.x, q.y, q.z), c.r2_filter);
             }
         } else {
             for (uint32_t k = 0; k < cnt; ++k) {
                 const float4 q = pb[k];
-                word = push_bit(word, dist2_exact(px, py, pz, q.x, q.y, q.z), c.r2);
+                word = push_bit(word, dist2_filter(px, py, pz, q.x, q.y, q.z), c.r2_filter);
             }
             word <<= (32u - cnt);
         }
@@ -455,15 +468,16 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
         if (nw == 0) break;
         wave_sync();
         store_words(item, seg_base, nw, lmask, lwj0, gmask, gwj0, lane);
-        cnt += sl.count;
         BitWalk<kSeg> bw;
         bw.ne = sl.nonempty;
         for (uint32_t t = 0; t < sl.count; ++t) {
             const uint32_t j = bw.next(lmask, lwj0, lane);
             const float4 q = spm[j];
             const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
+            const bool nb = d2 <= c.r2;  // the reference's test
+            cnt += nb ? 1u : 0u;
             const float w = w_shape(__builtin_amdgcn_sqrtf(d2) * c.inv_h);
-            acc += (j != s) ? q.w * w : 0.0f;
+            acc += (nb && j != s) ? q.w * w : 0.0f;
         }
         seg_base += nw;
         wave_sync();
@@ -611,7 +625,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
             const float dwdq = inner ? q * (-2.0f + 1.5f * q) : -0.5f * t2 * t2;
             const float lapf = inner ? (-2.0f + 3.0f * q) : t2;
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const bool other = j != s;
+            const bool other = j != s && d2 <= c.r2;  // the reference's test, self excluded
             const float P_j = c.k_gas * (vj.w - c.rho0);
             // pressure: skipped for |r| < 1e-6 (and self); viscosity: skipped for self only
             const float coef = (other && rl >= 1e-6f)
@@ -711,12 +725,16 @@ __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
             phase1_segment<kSeg>(cu, ch, spm, cs, c, pi.x, pi.y, pi.z, lmask, lwj0, lane, sl);
         if (nw == 0) break;
         wave_sync();
-        if (pass == 1) {
-            BitWalk<kSeg> bw;
-            bw.ne = sl.nonempty;
-            for (uint32_t t = 0; t < sl.count; ++t) idx[o++] = sid[bw.next(lmask, lwj0, lane)];
+        BitWalk<kSeg> bw;
+        bw.ne = sl.nonempty;
+        for (uint32_t t = 0; t < sl.count; ++t) {
+            const uint32_t j = bw.next(lmask, lwj0, lane);
+            const float4 q = spm[j];
+            if (dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z) <= c.r2) {
+                if (pass == 1) idx[o++] = sid[j];
+                ++cnt;
+            }
         }
-        cnt += sl.count;
         wave_sync();
         if (cu.done) break;
     }
