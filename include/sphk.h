@@ -187,6 +187,95 @@ int sphk_get_walk_shapes(const sphk_engine* e, int* density_shape, int* force_sh
 int sphk_walk_shape_count(void);
 const char* sphk_walk_shape_name(int shape);
 
+/* ---- learned macro-step surrogate ------------------------------------------------------
+ * No counterpart in the reference (its repository holds no learned component).  A
+ * message-passing MLP over the step's own neighbour lists predicts the acceleration of one
+ * macro-step of `substeps` plain steps:
+ *   edge e_ij = [r_ij/R, (v_i - v_j)/v_s, |r_ij|/R, W(|r_ij|)/W(0)]     (8)
+ *   S_i = c sum_{j != i} ReLU(W1 e_ij + b1)      m_i = W2 S_i + c n_i b2     (128, 128)
+ *   z_i = ReLU(Wn1 [m_i, (rho_i - rho0) k_rho, v_i / v_s] + bn1)           (132 -> 128)
+ *   a_i = a_s (Wn2 z_i + bn2)                                                (3)
+ * The macro-step over T = substeps * dt is v' = d^substeps v + a T, x' = x + T (v + v') / 2
+ * followed by the wall clamp (d = damping).  Training targets are
+ * a* = (v(t+T) - d^substeps v(t)) / T from the engine's own plain steps.  Dense layers run
+ * on bf16 MFMA with FP32 accumulation; parameters are FP32 (Adam) and rounded to bf16 for
+ * every use. */
+typedef struct {
+    int32_t substeps;     /* plain steps replaced by one macro-step (4) */
+    int32_t judge_every;  /* macro-steps per judge run (16) */
+    float tolerance;      /* fallback above this mean |rho - rho_SPH| in kg/m^3 (10 = 1% of 1000) */
+    float inv_v;          /* 1/v_s, velocity feature scale (1 s/m) */
+    float rho_scale;      /* k_rho, density feature scale (0.01 m^3/kg) */
+    float msg_scale;      /* c, message sum scale (1/64) */
+    float acc_scale;      /* a_s, output scale (10 m/s^2) */
+} sphk_surrogate_config;
+
+/* Training on trajectories the engine itself generates.  Each snapshot is one macro-step
+ * of plain SPH from the current state: its targets come from the plain run, then
+ * `iters_per_snapshot` Adam steps train on random batches of chunks (64 cell-sorted
+ * particles each) of the snapshot's starting state, and the run continues from the plain
+ * result.  The engine ends `snapshots` macro-steps later in simulated time. */
+typedef struct {
+    uint64_t snapshots;          /* 2000 */
+    uint32_t iters_per_snapshot; /* 1 */
+    uint32_t batch_chunks;       /* 1024 chunks = 65,536 particles */
+    float dt;                    /* plain step of the generating run */
+    float lr;                    /* Adam learning rate (1e-3; beta 0.9, 0.999, eps 1e-8) */
+    uint64_t seed;               /* batch selection */
+} sphk_train_config;
+
+/* Outcome of sphk_surrogate_run. */
+typedef struct {
+    uint64_t macro_steps;        /* macro-steps advanced by the network */
+    uint64_t plain_macro_steps;  /* macro-steps advanced by plain SPH (judge runs, fallback) */
+    uint64_t judges;             /* judge runs */
+    float last_error;            /* mean |rho - rho_SPH| of the last judge run, kg/m^3 */
+    float max_error;             /* largest mean error of any judge run */
+    float max_abs_error;         /* largest per-particle |rho - rho_SPH| of any judge run */
+    int32_t fell_back;           /* 1 once the run switched to plain SPH */
+    uint64_t fallback_at;        /* macro-step index of the switch */
+} sphk_surrogate_report;
+
+int sphk_surrogate_default_config(sphk_surrogate_config* out);
+/* Number of FP32 parameters (35,075). */
+int sphk_surrogate_param_count(size_t* out);
+/* Creates (or resets) the engine's network: Xavier-uniform weights from `seed`, zero
+ * biases, zero Adam state.  NULL cfg = defaults. */
+int sphk_surrogate_init(sphk_engine* e, const sphk_surrogate_config* cfg, uint64_t seed);
+int sphk_surrogate_get_config(sphk_engine* e, sphk_surrogate_config* out);
+/* Flat parameters W1[128][8] b1[128] W2[128][128] b2[128] Wn1[128][132] bn1[128]
+ * Wn2[3][128] bn2[3].  set_params resets the Adam state. */
+int sphk_surrogate_get_params(sphk_engine* e, float* out, size_t count);
+int sphk_surrogate_set_params(sphk_engine* e, const float* params, size_t count);
+/* Network acceleration (m/s^2) of every particle of the current state, caller order,
+ * count >= 3 n floats.  The state is not advanced. */
+int sphk_surrogate_predict(sphk_engine* e, float* out, size_t count);
+/* Targets of the current state: advances the engine by one macro-step of plain SPH at dt
+ * and writes a* (m/s^2, caller order, 3 n floats). */
+int sphk_surrogate_targets(sphk_engine* e, float dt, float* out, size_t count);
+/* Mean-square loss (normalised units) and its gradient for the given targets (m/s^2, caller
+ * order) on a batch of `batch_chunks` random chunks of the current state (all chunks when
+ * larger than their number).  in_batch (optional, n bytes) marks the batch particles. */
+int sphk_surrogate_loss_grad(sphk_engine* e, const float* target, size_t count,
+                             uint64_t batch_seed, uint32_t batch_chunks, float* loss,
+                             float* grad, size_t grad_count, uint8_t* in_batch);
+/* One Adam step with the given gradient (count = parameter count). */
+int sphk_surrogate_adam(sphk_engine* e, const float* grad, size_t count, float lr);
+/* Generates and trains (see sphk_train_config).  loss_curve (optional) receives the mean
+ * batch loss of each snapshot, up to curve_len values. */
+int sphk_surrogate_train(sphk_engine* e, const sphk_train_config* tc, float* loss_curve,
+                         size_t curve_len);
+/* One macro-step of the network (no judge). */
+int sphk_surrogate_macro_step(sphk_engine* e, float dt);
+/* Runs macro_steps macro-steps with the judge: every judge_every-th macro-step is run by
+ * plain SPH and by the network from the same state; when the mean |rho - rho_SPH| of the
+ * two results exceeds the tolerance the run continues with plain SPH only.  The plain
+ * result is kept at every judge run. */
+int sphk_surrogate_run(sphk_engine* e, uint64_t macro_steps, float dt,
+                       sphk_surrogate_report* rep);
+/* Densities of the current state (one density pass, caller order, n floats). */
+int sphk_surrogate_densities(sphk_engine* e, float* out, size_t count);
+
 /* Last error message of the calling thread; returns its length. */
 int sphk_last_error(char* buf, size_t len);
 /* Library version string ("1.0.0", the reference module's __version__). */

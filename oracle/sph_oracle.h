@@ -117,6 +117,36 @@ size_t    or_brute_neighbors(size_t n, const float* pos, float r2, int64_t* offs
 
 int       or_num_threads(void);
 
+/* ---- learned macro-step surrogate (surrogate_oracle.c) ---- */
+typedef struct {
+    int32_t h1, h2, h3; /* widths: message hidden, message output, node hidden */
+    int32_t bf16;       /* 1: round to bfloat16 where the device does */
+    float inv_r;        /* 1/R, scale of r and |r| */
+    float inv_v;        /* 1/v_s, scale of velocities */
+    float inv_h;        /* 1/h, kernel argument q = |r|/h */
+    float w_norm;       /* 1/w_shape(0) */
+    float rho0;         /* rest density */
+    float rho_scale;    /* node feature (rho - rho0) * rho_scale */
+    float msg_scale;    /* S_i = msg_scale * sum_j ReLU(u_ij) */
+    float acc_scale;    /* a = acc_scale * out */
+} or_sur_cfg;
+
+float     or_bf16_round(float f);
+size_t    or_sur_nparams(const or_sur_cfg* c);
+/* Network output (normalised units, n*3) over CSR neighbour lists that may include self.
+ * S_out (n*h1) and count_out (n) are optional. */
+void      or_sur_forward(const or_sur_cfg* c, const float* params, size_t n, const float* pos,
+                         const float* vel, const float* rho, const int64_t* off,
+                         const int64_t* idx, double* out, double* S_out, int64_t* count_out);
+/* Mean-square loss over the particles with in_batch[i] != 0 and its gradient (np values). */
+double    or_sur_loss_grad(const or_sur_cfg* c, const float* params, size_t n, const float* pos,
+                           const float* vel, const float* rho, const int64_t* off,
+                           const int64_t* idx, const uint8_t* in_batch, const float* target,
+                           double* grad);
+/* One Adam step (t >= 1) in place, FP64 arithmetic. */
+void      or_sur_adam(size_t np, float* w, float* m, float* v, const float* g, float lr,
+                      float beta1, float beta2, float eps, int32_t t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,16 @@ class Params(C.Structure):
     ]
 
 
+class SurCfg(C.Structure):
+    """Mirror of or_sur_cfg (surrogate network definition, see surrogate_oracle.c)."""
+    _fields_ = [
+        ("h1", C.c_int32), ("h2", C.c_int32), ("h3", C.c_int32), ("bf16", C.c_int32),
+        ("inv_r", C.c_float), ("inv_v", C.c_float), ("inv_h", C.c_float),
+        ("w_norm", C.c_float), ("rho0", C.c_float), ("rho_scale", C.c_float),
+        ("msg_scale", C.c_float), ("acc_scale", C.c_float),
+    ]
+
+
 _F = C.POINTER(C.c_float)
 _D = C.POINTER(C.c_double)
 _I64 = C.POINTER(C.c_int64)
@@ -71,6 +81,14 @@ _SIG = {
     "or64_accel_term_scale": (None, [_SZ, _F, _F, _F, _F, _I64, _I64, C.POINTER(Params), _D]),
     "or_brute_neighbors": (_SZ, [_SZ, _F, C.c_float, _I64, _I64, _SZ]),
     "or_num_threads": (C.c_int, []),
+    "or_bf16_round": (C.c_float, [C.c_float]),
+    "or_sur_nparams": (_SZ, [C.POINTER(SurCfg)]),
+    "or_sur_forward": (None, [C.POINTER(SurCfg), _F, _SZ, _F, _F, _F, _I64, _I64, _D, _D,
+                              _I64]),
+    "or_sur_loss_grad": (C.c_double, [C.POINTER(SurCfg), _F, _SZ, _F, _F, _F, _I64, _I64,
+                                      C.POINTER(C.c_uint8), _F, _D]),
+    "or_sur_adam": (None, [_SZ, _F, _F, _F, _F, C.c_float, C.c_float, C.c_float, C.c_float,
+                           C.c_int32]),
 }
 
 _libs = {}
@@ -231,3 +249,71 @@ def brute_neighbors(pos, r2):
     load().or_brute_neighbors(n, _f(pos), float(r2), offs.ctypes.data_as(_I64),
                               idx.ctypes.data_as(_I64), tot)
     return offs, idx[:tot]
+
+
+# ---------------------------------------------------------------- surrogate network
+def sur_cfg(h1=128, h2=128, h3=128, bf16=True, inv_r=25.0, inv_v=1.0, inv_h=50.0,
+            w_norm=1.5, rho0=1000.0, rho_scale=0.01, msg_scale=1.0 / 64.0,
+            acc_scale=10.0) -> SurCfg:
+    return SurCfg(h1, h2, h3, int(bf16), inv_r, inv_v, inv_h, w_norm, rho0, rho_scale,
+                  msg_scale, acc_scale)
+
+
+def sur_nparams(cfg: SurCfg) -> int:
+    return int(load().or_sur_nparams(C.byref(cfg)))
+
+
+def bf16_round(x) -> np.ndarray:
+    """Round-to-nearest-even to bfloat16, returned as float32 (vectorised restatement of
+    or_bf16_round; the two are compared in the tests)."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    nan = ((u & 0x7F800000) == 0x7F800000) & ((u & 0x007FFFFF) != 0)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    r = np.where(nan, (u | 0x00400000) & 0xFFFF0000, r)
+    return r.astype(np.uint32).view(np.float32)
+
+
+def _csr(offs, idx):
+    return (np.ascontiguousarray(offs, np.int64), np.ascontiguousarray(idx, np.int64))
+
+
+def sur_forward(cfg: SurCfg, params, pos, vel, rho, offs, idx, want_S=False):
+    """Network output (n, 3) in normalised units; optionally S (n, h1) and edge counts."""
+    params = np.ascontiguousarray(params, np.float32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+    rho = np.ascontiguousarray(rho, np.float32)
+    offs, idx = _csr(offs, idx)
+    n = pos.shape[0]
+    out = np.empty((n, 3), np.float64)
+    S = np.empty((n, cfg.h1), np.float64) if want_S else None
+    cnt = np.empty(n, np.int64) if want_S else None
+    load().or_sur_forward(C.byref(cfg), _f(params), n, _f(pos), _f(vel), _f(rho),
+                          offs.ctypes.data_as(_I64), idx.ctypes.data_as(_I64),
+                          out.ctypes.data_as(_D), S.ctypes.data_as(_D) if want_S else None,
+                          cnt.ctypes.data_as(_I64) if want_S else None)
+    return (out, S, cnt) if want_S else out
+
+
+def sur_loss_grad(cfg: SurCfg, params, pos, vel, rho, offs, idx, in_batch, target):
+    """(loss, gradient) in FP64 over the particles with in_batch != 0."""
+    params = np.ascontiguousarray(params, np.float32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+    rho = np.ascontiguousarray(rho, np.float32)
+    target = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+    mask = np.ascontiguousarray(in_batch, np.uint8)
+    offs, idx = _csr(offs, idx)
+    n = pos.shape[0]
+    grad = np.empty(params.size, np.float64)
+    loss = load().or_sur_loss_grad(C.byref(cfg), _f(params), n, _f(pos), _f(vel), _f(rho),
+                                   offs.ctypes.data_as(_I64), idx.ctypes.data_as(_I64),
+                                   mask.ctypes.data_as(C.POINTER(C.c_uint8)), _f(target),
+                                   grad.ctypes.data_as(_D))
+    return float(loss), grad
+
+
+def sur_adam(w, m, v, g, t, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam step in place on float32 arrays w, m, v (FP64 arithmetic)."""
+    g = np.ascontiguousarray(g, np.float32)
+    load().or_sur_adam(w.size, _f(w), _f(m), _f(v), _f(g), lr, beta1, beta2, eps, int(t))

@@ -26,7 +26,8 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SPHK_SOURCES = ["csrc/sphk_kernels.hip", "csrc/sphk_walk.hip", "csrc/sphk_api.hip"]
+SPHK_SOURCES = ["csrc/sphk_kernels.hip", "csrc/sphk_walk.hip", "csrc/sphk_api.hip",
+                "csrc/sphk_surrogate.hip", "csrc/sphk_surrogate_api.hip"]
 ENGINE_SOURCES = ["host/particle.cpp", "host/sph_engine.cpp"]
 
 

@@ -10,11 +10,11 @@
 #include <string>
 #include <vector>
 
-#include "sphk_internal.h"
+#include "sphk_engine.h"
 
 using namespace sphk;
 
-namespace {
+namespace sphk {
 
 thread_local std::string g_last_error;
 
@@ -28,122 +28,21 @@ int set_error(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                      \
-    do {                                                                                   \
-        hipError_t _e = (expr);                                                            \
-        if (_e != hipSuccess)                                                              \
-            return set_error(SPHK_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e));   \
-    } while (0)
+const std::string& last_error_text() { return g_last_error; }
+void set_last_error_text(const std::string& s) { g_last_error = s; }
+
+}  // namespace sphk
+
+namespace {
 
 // Largest dense grid (cells incl. padding) an engine will build.  4 bytes per cell.
 constexpr uint64_t kMaxCells = 1ull << 28;
 // Cell coordinates beyond +-2^30 cannot be represented in the dense grid.
 constexpr double kMaxCellCoord = 1073741824.0;
 
-template <class T>
-hipError_t dalloc(T** p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
-}
-
 }  // namespace
 
-// Host view of one scene: particle range, wall bounds, the box its finite positions may
-// occupy, and its dense grid.
-struct SceneHost {
-    uint64_t begin = 0, end = 0;
-    float bounds[6] = {-1.0f, 1.0f, -1.0f, 1.0f, -1.0f, 1.0f};  // xmin xmax ymin ymax zmin zmax
-    float ext_lo[3] = {0, 0, 0}, ext_hi[3] = {0, 0, 0};
-    bool ext_any = false;
-    Grid grid;
-};
-
-// Everything a captured step depends on besides buffer contents.
-struct GraphKey {
-    StepConsts consts;
-    uint32_t n;
-    int32_t shape_density, shape_force;
-    int32_t adaptive;
-    float dt;
-    uint64_t buffer_gen;
-};
-
-struct sphk_engine {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    size_t cap = 0;
-    size_t n = 0;
-    sphk_params params{};
-    uint64_t step_count = 0;
-
-    // home arrays (caller order)
-    float4* pm = nullptr;
-    float4* vel = nullptr;
-    float4* acc = nullptr;
-    float* rho = nullptr;
-    float* prs = nullptr;
-    // snapshot
-    float4* snap_pm = nullptr;
-    float4* snap_vel = nullptr;
-    float4* snap_acc = nullptr;
-    size_t snap_n = 0;
-    uint64_t snap_steps = 0;
-    std::vector<SceneHost> snap_scenes;
-    bool snap_ext_unknown = false;
-    bool has_snapshot = false;
-    // work arrays
-    uint32_t* keys = nullptr;
-    uint32_t* keys_alt = nullptr;
-    uint32_t* vals = nullptr;
-    uint32_t* vals_alt = nullptr;
-    uint32_t* hist = nullptr;
-    uint32_t* scan_scratch = nullptr;
-    size_t scan_scratch_cap = 0;
-    WalkBuffers walk;
-    int shape_density = 0;  // walk launch shapes (walk_shape_name), set by sphk_set_walk_shapes
-    int shape_force = 0;
-    float4* spm = nullptr;
-    float4* svr = nullptr;
-    uint32_t* cell_start = nullptr;
-    uint64_t cell_start_cap = 0;
-    uint32_t* skey = nullptr;  // -> keys or keys_alt after the sort
-    uint32_t* sid = nullptr;   // -> vals or vals_alt after the sort
-    // device scalars: [0] dt, [1] time, [2] snapshot time
-    float* fscal = nullptr;
-    // [0] out-of-grid flag, [1] cfl scratch
-    uint32_t* uscal = nullptr;
-    // [0] max neighbours, [1] total neighbours
-    unsigned long long* nstats = nullptr;
-
-    // scenes and their grids
-    std::vector<SceneHost> scenes;  // one implicit scene [0, n) unless set by sphk_set_scenes
-    bool scenes_explicit = false;
-    bool ext_unknown = false;  // positions may lie outside the scene extents (device bbox needed)
-    bool grid_valid = false;
-    std::vector<SceneGrid> table;  // host copy of the device scene table
-    SceneGrid* d_scenes = nullptr;
-    size_t d_scenes_cap = 0;
-    StepConsts consts{};
-    bool sorted_valid = false;  // work arrays hold the cell-sorted image of the current state
-    uint64_t buffer_gen = 0;    // bumped when a buffer used by the step is reallocated
-
-    // captured step
-    bool use_graph = false;
-    bool graph_valid = false;
-    GraphKey graph_key{};
-    hipGraphExec_t graph_exec = nullptr;
-
-    // timing: one event set per timed step, drained when stats are read
-    struct EventSet {
-        hipEvent_t ev[4];
-    };
-    bool timing = false;
-    std::vector<EventSet> pending, free_sets;
-    sphk_stats stats{};
-};
-
-namespace {
+namespace sphk {
 
 void fill_default(sphk_params* p) {
     p->rest_density = 1000.0f;
@@ -509,7 +408,111 @@ int set_device(const sphk_engine* e) {
     return SPHK_OK;
 }
 
-}  // namespace
+}  // namespace sphk
+
+namespace sphk {
+
+int prepare_grid(sphk_engine* e) {
+    if (!e->grid_valid || e->ext_unknown || !e->scenes_explicit) {
+        if (int rc = check_scenes(e)) return rc;
+    }
+    if (!e->grid_valid || e->ext_unknown) {
+        if (int rc = ensure_grid(e)) return rc;
+    }
+    return SPHK_OK;
+}
+
+void after_positions_moved(sphk_engine* e) {
+    e->sorted_valid = false;
+    // after the wall clamp every finite position of a scene lies in its bounds box
+    bool changed = false;
+    for (auto& sc : e->scenes) {
+        float lo[3], hi[3];
+        if (bounds_box(sc.bounds, lo, hi) != 0) {
+            e->ext_unknown = true;
+            continue;
+        }
+        SceneHost next = sc;
+        next.ext_any = true;
+        for (int d = 0; d < 3; ++d) {
+            next.ext_lo[d] = lo[d];
+            next.ext_hi[d] = hi[d];
+        }
+        std::string keep = g_last_error;
+        Grid g;
+        if (grid_for_ext(next, e->params.neighbor_search_radius, &g) != SPHK_OK) {
+            g_last_error = keep;  // walls too wide to bound the grid: measure the live bbox
+            e->ext_unknown = true;
+            continue;
+        }
+        if (g.G != sc.grid.G || g.ox != sc.grid.ox || g.oy != sc.grid.oy || g.oz != sc.grid.oz ||
+            g.nx != sc.grid.nx || g.ny != sc.grid.ny || g.nz != sc.grid.nz)
+            changed = true;
+        for (int d = 0; d < 3; ++d) {
+            sc.ext_lo[d] = lo[d];
+            sc.ext_hi[d] = hi[d];
+        }
+        sc.ext_any = true;
+    }
+    if (changed) e->grid_valid = false;
+}
+
+int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev) {
+    hipStream_t st = e->stream;
+    const uint32_t n = (uint32_t)e->n;
+    if (ev) HIP_TRY(hipEventRecord(ev[0], st));
+    if (adaptive) HIP_TRY(launch_cfl(e->vel, e->acc, n, e->params, e->uscal + 1, e->fscal, st));
+    if (int rc = build_sorted(e)) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[1], st));
+    HIP_TRY(launch_density_walk(e->walk, e->spm, e->svr, e->skey, e->cell_start, n, e->consts,
+                                e->d_scenes, e->nstats, e->shape_density, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], st));
+    HIP_TRY(launch_force_walk(e->walk, e->spm, e->svr, e->skey, e->sid, e->cell_start, n,
+                              e->consts, e->d_scenes, adaptive ? e->fscal : nullptr, dt, e->pm,
+                              e->vel, e->acc, e->rho, e->prs, e->shape_force, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[3], st));
+    HIP_TRY(launch_advance_time(e->fscal + 1, adaptive ? e->fscal : nullptr, dt, st));
+    return SPHK_OK;
+}
+
+// Captures the step into a graph (or reuses the one captured for the same key) and
+// launches it.
+int graph_step(sphk_engine* e, float dt, bool adaptive) {
+    GraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.consts = e->consts;
+    key.n = (uint32_t)e->n;
+    key.shape_density = e->shape_density;
+    key.shape_force = e->shape_force;
+    key.adaptive = adaptive;
+    key.dt = adaptive ? 0.0f : dt;
+    key.buffer_gen = e->buffer_gen;
+    if (!e->graph_valid || memcmp(&key, &e->graph_key, sizeof(key)) != 0) {
+        if (e->graph_exec) {
+            HIP_TRY(hipGraphExecDestroy(e->graph_exec));
+            e->graph_exec = nullptr;
+        }
+        e->graph_valid = false;
+        HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(e, dt, adaptive, nullptr);
+        hipGraph_t graph = nullptr;
+        hipError_t ec = hipStreamEndCapture(e->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIP_TRY(ec);
+        hipError_t ei = hipGraphInstantiate(&e->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(ei);
+        memcpy(&e->graph_key, &key, sizeof(key));
+        e->graph_valid = true;
+    }
+    HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
+    return SPHK_OK;
+}
+
+}  // namespace sphk
 
 extern "C" {
 
@@ -578,6 +581,7 @@ int sphk_destroy(sphk_engine* e) {
                     (void*)e->walk.mask, (void*)e->walk.wj0, (void*)e->walk.nwords,
                     (void*)e->scan_scratch, (void*)e->d_scenes})
         if (p) (void)hipFree(p);
+    if (e->sur) surrogate_free(e->sur);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     for (auto* v : {&e->pending, &e->free_sets})
         for (auto& set : *v)
@@ -747,71 +751,11 @@ int sphk_clear(sphk_engine* e) {
 }
 
 // Enqueues the kernels of one step on the engine's stream (eagerly or into a capture).
-int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev) {
-    hipStream_t st = e->stream;
-    const uint32_t n = (uint32_t)e->n;
-    if (ev) HIP_TRY(hipEventRecord(ev[0], st));
-    if (adaptive) HIP_TRY(launch_cfl(e->vel, e->acc, n, e->params, e->uscal + 1, e->fscal, st));
-    if (int rc = build_sorted(e)) return rc;
-    if (ev) HIP_TRY(hipEventRecord(ev[1], st));
-    HIP_TRY(launch_density_walk(e->walk, e->spm, e->svr, e->skey, e->cell_start, n, e->consts,
-                                e->d_scenes, e->nstats, e->shape_density, st));
-    if (ev) HIP_TRY(hipEventRecord(ev[2], st));
-    HIP_TRY(launch_force_walk(e->walk, e->spm, e->svr, e->skey, e->sid, e->cell_start, n,
-                              e->consts, e->d_scenes, adaptive ? e->fscal : nullptr, dt, e->pm,
-                              e->vel, e->acc, e->rho, e->prs, e->shape_force, st));
-    if (ev) HIP_TRY(hipEventRecord(ev[3], st));
-    HIP_TRY(launch_advance_time(e->fscal + 1, adaptive ? e->fscal : nullptr, dt, st));
-    return SPHK_OK;
-}
-
-// Captures the step into a graph (or reuses the one captured for the same key) and
-// launches it.
-int graph_step(sphk_engine* e, float dt, bool adaptive) {
-    GraphKey key;
-    memset(&key, 0, sizeof(key));
-    key.consts = e->consts;
-    key.n = (uint32_t)e->n;
-    key.shape_density = e->shape_density;
-    key.shape_force = e->shape_force;
-    key.adaptive = adaptive;
-    key.dt = adaptive ? 0.0f : dt;
-    key.buffer_gen = e->buffer_gen;
-    if (!e->graph_valid || memcmp(&key, &e->graph_key, sizeof(key)) != 0) {
-        if (e->graph_exec) {
-            HIP_TRY(hipGraphExecDestroy(e->graph_exec));
-            e->graph_exec = nullptr;
-        }
-        e->graph_valid = false;
-        HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(e, dt, adaptive, nullptr);
-        hipGraph_t graph = nullptr;
-        hipError_t ec = hipStreamEndCapture(e->stream, &graph);
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc;
-        }
-        HIP_TRY(ec);
-        hipError_t ei = hipGraphInstantiate(&e->graph_exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        HIP_TRY(ei);
-        memcpy(&e->graph_key, &key, sizeof(key));
-        e->graph_valid = true;
-    }
-    HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
-    return SPHK_OK;
-}
-
 int sphk_step(sphk_engine* e, float dt) {
     if (int rc = check_engine(e)) return rc;
     if (e->n == 0) return SPHK_OK;  // SPHEngine::step returns when empty (src/sph_engine.cpp:94)
     if (int rc = set_device(e)) return rc;
-    if (!e->grid_valid || e->ext_unknown || !e->scenes_explicit) {
-        if (int rc = check_scenes(e)) return rc;
-    }
-    if (!e->grid_valid || e->ext_unknown) {
-        if (int rc = ensure_grid(e)) return rc;
-    }
+    if (int rc = prepare_grid(e)) return rc;
     const bool adaptive = !(dt > 0.0f);
     hipEvent_t* ev = nullptr;
     // bound the events waiting to be read: fold the oldest in every 256 timed steps
@@ -833,41 +777,10 @@ int sphk_step(sphk_engine* e, float dt) {
     } else {
         if (int rc = enqueue_step(e, dt, adaptive, ev)) return rc;
     }
-    e->sorted_valid = false;
     e->step_count++;
     e->stats.total_neighbor_queries += e->n;
     e->stats.steps++;
-    // after the wall clamp every finite position of a scene lies in its bounds box
-    bool changed = false;
-    for (auto& sc : e->scenes) {
-        float lo[3], hi[3];
-        if (bounds_box(sc.bounds, lo, hi) != 0) {
-            e->ext_unknown = true;
-            continue;
-        }
-        SceneHost next = sc;
-        next.ext_any = true;
-        for (int d = 0; d < 3; ++d) {
-            next.ext_lo[d] = lo[d];
-            next.ext_hi[d] = hi[d];
-        }
-        std::string keep = g_last_error;
-        Grid g;
-        if (grid_for_ext(next, e->params.neighbor_search_radius, &g) != SPHK_OK) {
-            g_last_error = keep;  // walls too wide to bound the grid: measure the live bbox
-            e->ext_unknown = true;
-            continue;
-        }
-        if (g.G != sc.grid.G || g.ox != sc.grid.ox || g.oy != sc.grid.oy || g.oz != sc.grid.oz ||
-            g.nx != sc.grid.nx || g.ny != sc.grid.ny || g.nz != sc.grid.nz)
-            changed = true;
-        for (int d = 0; d < 3; ++d) {
-            sc.ext_lo[d] = lo[d];
-            sc.ext_hi[d] = hi[d];
-        }
-        sc.ext_any = true;
-    }
-    if (changed) e->grid_valid = false;
+    after_positions_moved(e);
     return SPHK_OK;
 }
 

@@ -144,7 +144,9 @@ hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint
                             int shape, hipStream_t st);
 hipError_t launch_cfl(const float4* vel, const float4* acc, uint32_t n, const sphk_params& p,
                       uint32_t* scratch_u32, float* dt_dev, hipStream_t st);
-hipError_t launch_advance_time(float* time_dev, const float* dt_dev, float dt, hipStream_t st);
+// time += dt (or *dt_dev), `reps` times in sequence (the clock of `reps` steps)
+hipError_t launch_advance_time(float* time_dev, const float* dt_dev, float dt, hipStream_t st,
+                               int reps = 1);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 
 }  // namespace sphk

@@ -380,13 +380,16 @@ hipError_t launch_cfl(const float4* vel, const float4* acc, uint32_t n, const sp
 }
 
 __global__ void k_advance_time(float* __restrict__ t, const float* __restrict__ dt_dev,
-                               float dt) {
+                               float dt, int reps) {
     float d = dt_dev ? *dt_dev : dt;
-    *t = *t + d;
+    float x = *t;
+    for (int k = 0; k < reps; ++k) x = x + d;
+    *t = x;
 }
 
-hipError_t launch_advance_time(float* time_dev, const float* dt_dev, float dt, hipStream_t st) {
-    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, st, time_dev, dt_dev, dt);
+hipError_t launch_advance_time(float* time_dev, const float* dt_dev, float dt, hipStream_t st,
+                               int reps) {
+    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, st, time_dev, dt_dev, dt, reps);
     return hipGetLastError();
 }
 

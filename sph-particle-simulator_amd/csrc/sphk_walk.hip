@@ -21,357 +21,9 @@
 
 #include <climits>
 
-#include "sphk_internal.h"
+#include "sphk_walk_common.h"
 
 namespace sphk {
-
-namespace {
-
-// LDS visibility between lanes of one wavefront: LDS operations of a wave complete in
-// order, so only the compiler must be kept from reordering them.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ float next_up(float x) {
-    if (x == 0.0f) return __int_as_float(1);
-    int i = __float_as_int(x);
-    return __int_as_float(x > 0.0f ? i + 1 : i - 1);
-}
-
-__device__ __forceinline__ float next_down(float x) {
-    if (x == 0.0f) return -__int_as_float(1);
-    int i = __float_as_int(x);
-    return __int_as_float(x > 0.0f ? i - 1 : i + 1);
-}
-
-// (dx*dx + dy*dy) + dz*dz with dx = a - b, every operation rounded once
-// (within_radius_squared / glm::dot, src/spatial_hash.h:70-73).
-__device__ __forceinline__ float dist2_exact(float ax, float ay, float az, float bx, float by,
-                                             float bz) {
-#pragma clang fp contract(off)
-    float dx = ax - bx;
-    float dy = ay - by;
-    float dz = az - bz;
-    float tx = dx * dx;
-    float ty = dy * dy;
-    float tz = dz * dz;
-    return (tx + ty) + tz;
-}
-
-// Contracted d^2 for the candidate filter: fl(dx^2 + fl(dy^2 + fl(dz^2))) with two fused
-// multiply-adds.  Both roundings stay within 3 units of 2^-24 of the true sum, so the two
-// values differ by under 7 units of 2^-24 relative, and
-// testing it against r2 raised by 8 ulp (StepConsts::r2_filter) keeps every pair the exact
-// test keeps; the few extra pairs are dropped by the exact test where the pair is used.
-__device__ __forceinline__ float dist2_filter(float ax, float ay, float az, float bx, float by,
-                                              float bz) {
-    const float dx = ax - bx;
-    const float dy = ay - by;
-    const float dz = az - bz;
-    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-}
-
-// Can a particle at coordinate x (cell c = floor(fl(x*inv))) have a neighbour in cell c+2
-// (bit 1) or c-2 (bit 0)?  A neighbour needs fl(fl(b-x)^2) <= r2 on this axis alone, and
-// fl(b-x) grows with b, so it suffices to test the float b nearest to x that lands in cell
-// c+2 (resp. c-2).  Exact; conservative (flag set) if the bracket search does not settle.
-__device__ __forceinline__ int axis_edge_flags(float x, float inv, float r2) {
-#pragma clang fp contract(off)
-    float c = floorf(x * inv);
-    int flags = 0;
-    {
-        float t = c + 2.0f;
-        float b = t / inv;
-        int ok = 0;
-        for (int k = 0; k < 6; ++k) {
-            if (b * inv >= t) {
-                ok = 1;
-                break;
-            }
-            b = next_up(b);
-        }
-        if (ok) {
-            for (int k = 0; k < 6; ++k) {
-                float bd = next_down(b);
-                if (bd * inv >= t) {
-                    b = bd;
-                } else {
-                    ok = 2;
-                    break;
-                }
-            }
-        }
-        float d = b - x;
-        if (ok != 2 || d * d <= r2) flags |= 2;
-    }
-    {
-        float t = c - 1.0f;
-        float b = t / inv;
-        int ok = 0;
-        for (int k = 0; k < 6; ++k) {
-            if (b * inv < t) {
-                ok = 1;
-                break;
-            }
-            b = next_down(b);
-        }
-        if (ok) {
-            for (int k = 0; k < 6; ++k) {
-                float bu = next_up(b);
-                if (bu * inv < t) {
-                    b = bu;
-                } else {
-                    ok = 2;
-                    break;
-                }
-            }
-        }
-        float d = x - b;
-        if (ok != 2 || d * d <= r2) flags |= 1;
-    }
-    return flags;
-}
-
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-
-constexpr uint32_t kXcds = 8;
-
-// Block index after the XCD swizzle (see grid_for).
-__device__ __forceinline__ uint32_t swizzled_block() {
-    const uint32_t per = gridDim.x / kXcds;
-    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-}
-__device__ __forceinline__ int unii(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// Cubic-spline shape without sigma (CubicSplineKernel::W_3d, src/kernels.cpp:58-66):
-// 2/3 - q^2 + q^3/2 on [0,1], (2-q)^3/6 on (1,2], 0 beyond; both pieces are computed and
-// one is selected.
-__device__ __forceinline__ float w_shape(float q) {
-    const float q2 = q * q;
-    const float win = (2.0f / 3.0f - q2) + 0.5f * q2 * q;
-    const float t = fmaxf(2.0f - q, 0.0f);
-    const float wout = (1.0f / 6.0f) * t * t * t;
-    return q <= 1.0f ? win : wout;
-}
-
-// Wave-uniform description of one chunk and its candidate columns.
-struct Chunk {
-    uint32_t s0, s_end;   // sorted slots [s0, s_end)
-    uint32_t scene;       // scene whose grid holds the chunk
-    uint32_t cell_base;   // that grid's first global cell
-    uint32_t ny, nz;      // that grid's dims (y, z)
-    int rx, ry;           // column (grid-relative)
-    int dx0, dx1, dy0, dy1;
-    int zlo, zhi;         // candidate z cells (inclusive), grid-relative
-};
-
-// Cursor over the candidate columns, in (dx, dy) lexicographic order.
-struct Cursor {
-    int dx, dy;
-    uint32_t j, end;  // current slot and end of the current column range
-    bool done;
-};
-
-__device__ __forceinline__ void column_range(const Chunk& ch, int dx, int dy,
-                                             const uint32_t* __restrict__ cs, const StepConsts& c,
-                                             uint32_t& a, uint32_t& b) {
-    uint32_t base = ch.cell_base +
-                    ((uint32_t)(ch.rx + dx) * ch.ny + (uint32_t)(ch.ry + dy)) * ch.nz;
-    a = uni(cs[base + (uint32_t)ch.zlo]);
-    b = uni(cs[base + (uint32_t)ch.zhi + 1u]);
-}
-
-__device__ __forceinline__ void cursor_begin(Cursor& cu, const Chunk& ch,
-                                             const uint32_t* __restrict__ cs, const StepConsts& c) {
-    cu.dx = ch.dx0;
-    cu.dy = ch.dy0;
-    cu.done = false;
-    column_range(ch, cu.dx, cu.dy, cs, c, cu.j, cu.end);
-}
-
-// Moves to the next column when the current one is exhausted; false when all are.
-__device__ __forceinline__ bool cursor_settle(Cursor& cu, const Chunk& ch,
-                                              const uint32_t* __restrict__ cs,
-                                              const StepConsts& c) {
-    while (cu.j >= cu.end) {
-        if (cu.dy < ch.dy1) {
-            ++cu.dy;
-        } else if (cu.dx < ch.dx1) {
-            ++cu.dx;
-            cu.dy = ch.dy0;
-        } else {
-            cu.done = true;
-            return false;
-        }
-        column_range(ch, cu.dx, cu.dy, cs, c, cu.j, cu.end);
-    }
-    return true;
-}
-
-// Appends one test result to a 32-bit word: word = 2*word + (d2 <= r2).  One compare into
-// VCC and one add-with-carry; the first candidate of a word ends up in bit 31.
-__device__ __forceinline__ uint32_t push_bit(uint32_t word, float d2, float r2) {
-    uint32_t out;
-    asm("v_cmp_ge_f32 vcc, %2, %3\n\tv_addc_co_u32 %0, vcc, %1, %1, vcc"
-        : "=v"(out)
-        : "v"(word), "s"(r2), "v"(d2)
-        : "vcc");
-    return out;
-}
-
-// Per-lane summary of one LDS segment: which words hold a neighbour (bit w) and how many
-// neighbours in total.
-struct SegLane {
-    uint32_t nonempty;
-    uint32_t count;
-};
-
-// Phase 1: fills up to kSeg words of filter bits into LDS (masks[w*64 + lane], wj0[w]): a
-// superset of the neighbours (see dist2_filter).  Candidate k of a word sits in bit 31-k
-// (words of fewer than 32 candidates are left-aligned).  Returns the number of words written.
-template <int kSeg>
-__device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
-                                              const float4* __restrict__ spm,
-                                              const uint32_t* __restrict__ cs,
-                                              const StepConsts& c, float px, float py, float pz,
-                                              uint32_t* __restrict__ lmask,
-                                              uint32_t* __restrict__ lwj0, uint32_t lane,
-                                              SegLane& sl) {
-    static_assert(kSeg <= 32, "one nonempty bit per word");
-    int w = 0;
-    sl.nonempty = 0;
-    sl.count = 0;
-    while (w < kSeg && cursor_settle(cu, ch, cs, c)) {
-        const uint32_t j0 = uni(cu.j);
-        const uint32_t cnt = uni(min(32u, cu.end - j0));
-        uint32_t word = 0;
-        const float4* __restrict__ pb = spm + j0;  // uniform base: immediate offsets below
-        if (cnt == 32u) {
-#pragma unroll
-            for (int k = 0; k < 32; ++k) {
-                const float4 q = pb[k];
-                word = push_bit(word, dist2_filter(px, py, pz, q.x, q.y, q.z), c.r2_filter);
-            }
-        } else {
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4 q = pb[k];
-                word = push_bit(word, dist2_filter(px, py, pz, q.x, q.y, q.z), c.r2_filter);
-            }
-            word <<= (32u - cnt);
-        }
-        lmask[w * 64 + lane] = word;
-        if (lane == 0) lwj0[w] = j0;
-        sl.nonempty |= (word != 0u ? 1u : 0u) << w;
-        sl.count += (uint32_t)__builtin_popcount(word);
-        cu.j = j0 + cnt;
-        ++w;
-    }
-    return w;
-}
-
-// Builds the wave-uniform chunk description; returns false for an out-of-range item.
-// Lanes past the chunk get NaN coordinates, so every test fails for them.
-__device__ __forceinline__ bool load_chunk(uint32_t item, const uint32_t* __restrict__ items,
-                                           const uint32_t* __restrict__ nitems,
-                                           const uint32_t* __restrict__ skey,
-                                           const uint32_t* __restrict__ cs,
-                                           const float4* __restrict__ spm, const StepConsts& c,
-                                           const SceneGrid* __restrict__ scenes, uint32_t lane,
-                                           Chunk& ch, uint32_t& s, bool& active, float4& pi) {
-    if (item >= uni(nitems[0])) return false;
-    ch.s0 = uni(items[item]);
-    const uint32_t L0 = uni(skey[ch.s0]);
-    ch.scene = uni(scene_of_cell(scenes, c.nscenes, L0));
-    ch.cell_base = uni(scenes[ch.scene].cell_base);
-    ch.ny = uni((uint32_t)scenes[ch.scene].ny);
-    ch.nz = uni((uint32_t)scenes[ch.scene].nz);
-    const uint32_t local = L0 - ch.cell_base;
-    const uint32_t col = local / ch.nz;
-    const uint32_t col_end = uni(cs[ch.cell_base + (col + 1u) * ch.nz]);
-    ch.s_end = min(ch.s0 + 64u, col_end);
-    ch.rx = (int)(col / ch.ny);
-    ch.ry = (int)(col % ch.ny);
-    const int zmin = (int)(local % ch.nz);
-    const int zmax = (int)((uni(skey[ch.s_end - 1u]) - ch.cell_base) % ch.nz);
-    s = ch.s0 + lane;
-    active = s < ch.s_end;
-    const float qnan = __int_as_float(0x7fc00000);
-    pi = active ? spm[s] : make_float4(qnan, qnan, qnan, 0.0f);
-    int fx = 0, fy = 0, fz = 0;
-    if (active) {
-        fx = axis_edge_flags(pi.x, c.inv_cell, c.r2);
-        fy = axis_edge_flags(pi.y, c.inv_cell, c.r2);
-        fz = axis_edge_flags(pi.z, c.inv_cell, c.r2);
-    }
-    const int fxl = __ballot(fx & 1) != 0, fxh = __ballot(fx & 2) != 0;
-    const int fyl = __ballot(fy & 1) != 0, fyh = __ballot(fy & 2) != 0;
-    const int fzl = __ballot(fz & 1) != 0, fzh = __ballot(fz & 2) != 0;
-    ch.dx0 = unii(-1 - fxl);
-    ch.dx1 = unii(1 + fxh);
-    ch.dy0 = unii(-1 - fyl);
-    ch.dy1 = unii(1 + fyh);
-    ch.zlo = unii(zmin - 1 - fzl);
-    ch.zhi = unii(zmax + 1 + fzh);
-    return true;
-}
-
-// Per-lane walk over the set bits of one segment, in candidate order.  Run exactly
-// SegLane::count times; refills from the next non-empty word by selects (no inner loop).
-// The word index read ahead stays inside the segment (kSeg - 1 when nothing is left).
-template <int kSeg>
-struct BitWalk {
-    uint32_t m = 0, jb = 0, ne = 0;
-    __device__ __forceinline__ uint32_t next(const uint32_t* __restrict__ lmask,
-                                             const uint32_t* __restrict__ lwj0, uint32_t lane) {
-        const uint32_t w = (uint32_t)__builtin_ctz(ne | (1u << (kSeg - 1)));
-        const uint32_t mn = lmask[w * 64 + lane];
-        const uint32_t jn = lwj0[w];
-        const bool need = m == 0u;
-        m = need ? mn : m;
-        jb = need ? jn : jb;
-        ne = need ? (ne & (ne - 1u)) : ne;
-        const uint32_t k = (uint32_t)__builtin_clz(m);
-        m ^= 0x80000000u >> k;
-        return jb + k;
-    }
-};
-
-__device__ __forceinline__ void add_nbr_stats(uint32_t cnt, unsigned long long* stats,
-                                              uint32_t lane) {
-    uint32_t mx = cnt;
-    unsigned long long sm = cnt;
-    for (int off = 32; off > 0; off >>= 1) {
-        uint32_t o = __shfl_xor(mx, off);
-        mx = o > mx ? o : mx;
-        sm += __shfl_xor(sm, off);
-    }
-    if (lane == 0) {
-        atomicMax(&stats[0], (unsigned long long)mx);
-        atomicAdd(&stats[1], sm);
-    }
-}
-
-// Stores the LDS words [0, nw) of segment `seg_base` of this chunk into HBM.
-__device__ __forceinline__ void store_words(uint32_t item, int seg_base, int nw,
-                                            const uint32_t* __restrict__ lmask,
-                                            const uint32_t* __restrict__ lwj0,
-                                            uint32_t* __restrict__ gmask,
-                                            uint32_t* __restrict__ gwj0, uint32_t lane) {
-    const size_t base = (size_t)item * kMaxWords;
-    for (int w = 0; w < nw; ++w) {
-        int gw = seg_base + w;
-        if (gw >= kMaxWords) break;
-        gmask[(base + gw) * 64 + lane] = lmask[w * 64 + lane];
-    }
-    for (int w = (int)lane; w < nw; w += 64) {
-        int gw = seg_base + w;
-        if (gw < kMaxWords) gwj0[base + gw] = lwj0[w];
-    }
-}
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // chunk list: per column, ceil(len/64) chunks of consecutive sorted slots
@@ -425,12 +77,6 @@ uint32_t max_chunks(uint32_t n, const StepConsts& c) { return (n + 63u) / 64u + 
 // Launch shapes of the walks: words per LDS segment x wavefronts per workgroup.  Every
 // wavefront works on its own chunk; the waves of a workgroup never synchronise.
 // ---------------------------------------------------------------------------
-template <int kSeg, int kWaves>
-struct WalkShape {
-    static constexpr int seg = kSeg;
-    static constexpr int waves = kWaves;
-    static constexpr int threads = 64 * kWaves;
-};
 
 // ---------------------------------------------------------------------------
 // density: rho_i = m_i W(0) + sum_{j != i} m_j W(r_ij)   (compute_density,

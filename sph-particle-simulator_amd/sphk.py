@@ -55,6 +55,34 @@ class Stats(C.Structure):
     ]
 
 
+class SurrogateConfig(C.Structure):
+    """sphk_surrogate_config: macro-step length, judge period/tolerance, feature scales."""
+
+    _fields_ = [
+        ("substeps", C.c_int32), ("judge_every", C.c_int32), ("tolerance", C.c_float),
+        ("inv_v", C.c_float), ("rho_scale", C.c_float), ("msg_scale", C.c_float),
+        ("acc_scale", C.c_float),
+    ]
+
+
+class TrainConfig(C.Structure):
+    """sphk_train_config: snapshots of the engine's own plain run used for training."""
+
+    _fields_ = [
+        ("snapshots", C.c_uint64), ("iters_per_snapshot", C.c_uint32),
+        ("batch_chunks", C.c_uint32), ("dt", C.c_float), ("lr", C.c_float),
+        ("seed", C.c_uint64),
+    ]
+
+
+class SurrogateReport(C.Structure):
+    _fields_ = [
+        ("macro_steps", C.c_uint64), ("plain_macro_steps", C.c_uint64), ("judges", C.c_uint64),
+        ("last_error", C.c_float), ("max_error", C.c_float), ("max_abs_error", C.c_float),
+        ("fell_back", C.c_int32), ("fallback_at", C.c_uint64),
+    ]
+
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _I32 = C.POINTER(C.c_int32)
@@ -99,6 +127,21 @@ SIGNATURES = {
     "sphk_get_walk_shapes": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "sphk_walk_shape_count": [],
     "sphk_walk_shape_name": [C.c_int],
+    "sphk_surrogate_default_config": [C.POINTER(SurrogateConfig)],
+    "sphk_surrogate_param_count": [C.POINTER(_SZ)],
+    "sphk_surrogate_init": [_P, C.POINTER(SurrogateConfig), C.c_uint64],
+    "sphk_surrogate_get_config": [_P, C.POINTER(SurrogateConfig)],
+    "sphk_surrogate_get_params": [_P, _F, _SZ],
+    "sphk_surrogate_set_params": [_P, _F, _SZ],
+    "sphk_surrogate_predict": [_P, _F, _SZ],
+    "sphk_surrogate_targets": [_P, C.c_float, _F, _SZ],
+    "sphk_surrogate_loss_grad": [_P, _F, _SZ, C.c_uint64, C.c_uint32, _F, _F, _SZ,
+                                 C.POINTER(C.c_uint8)],
+    "sphk_surrogate_adam": [_P, _F, _SZ, C.c_float],
+    "sphk_surrogate_train": [_P, C.POINTER(TrainConfig), _F, _SZ],
+    "sphk_surrogate_macro_step": [_P, C.c_float],
+    "sphk_surrogate_run": [_P, C.c_uint64, C.c_float, C.POINTER(SurrogateReport)],
+    "sphk_surrogate_densities": [_P, _F, _SZ],
     "sphk_last_error": [C.c_char_p, _SZ],
     "sphk_version": [],
 }
@@ -333,6 +376,93 @@ class Engine:
         g = C.c_uint64()
         check(self.lib.sphk_debug_grid(self.h, _ptr(o, C.c_int32), _ptr(d, C.c_int32), C.byref(g)))
         return o, d, g.value
+
+
+    # ---- learned macro-step surrogate ----
+    def surrogate_init(self, seed: int = 0, config: Optional[SurrogateConfig] = None):
+        check(self.lib.sphk_surrogate_init(self.h, C.byref(config) if config else None,
+                                           C.c_uint64(seed)))
+
+    def surrogate_config(self) -> SurrogateConfig:
+        c = SurrogateConfig()
+        check(self.lib.sphk_surrogate_get_config(self.h, C.byref(c)))
+        return c
+
+    def surrogate_params(self) -> np.ndarray:
+        out = np.empty(surrogate_param_count(), np.float32)
+        check(self.lib.sphk_surrogate_get_params(self.h, _ptr(out, C.c_float), out.size))
+        return out
+
+    def surrogate_set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        check(self.lib.sphk_surrogate_set_params(self.h, _ptr(p, C.c_float), p.size))
+
+    def surrogate_predict(self) -> np.ndarray:
+        n = self.size()
+        out = np.empty((n, 3), np.float32)
+        check(self.lib.sphk_surrogate_predict(self.h, _ptr(out, C.c_float), out.size))
+        return out
+
+    def surrogate_targets(self, dt: float) -> np.ndarray:
+        n = self.size()
+        out = np.empty((n, 3), np.float32)
+        check(self.lib.sphk_surrogate_targets(self.h, float(dt), _ptr(out, C.c_float), out.size))
+        return out
+
+    def surrogate_loss_grad(self, target, batch_seed: int = 0, batch_chunks: int = 1 << 30):
+        """(loss, gradient, in_batch mask) for targets in m/s^2 (caller order)."""
+        n = self.size()
+        t = np.ascontiguousarray(target, np.float32).reshape(n, 3)
+        loss = C.c_float()
+        grad = np.empty(surrogate_param_count(), np.float32)
+        mask = np.empty(n, np.uint8)
+        check(self.lib.sphk_surrogate_loss_grad(self.h, _ptr(t, C.c_float), t.size,
+                                                C.c_uint64(batch_seed), int(batch_chunks),
+                                                C.byref(loss), _ptr(grad, C.c_float), grad.size,
+                                                _ptr(mask, C.c_uint8)))
+        return loss.value, grad, mask
+
+    def surrogate_adam(self, grad, lr: float = 1e-3):
+        g = np.ascontiguousarray(grad, np.float32)
+        check(self.lib.sphk_surrogate_adam(self.h, _ptr(g, C.c_float), g.size, float(lr)))
+
+    def surrogate_train(self, snapshots: int, dt: float, batch_chunks: int = 1024,
+                        iters_per_snapshot: int = 1, lr: float = 1e-3, seed: int = 0):
+        """Trains on the engine's own trajectory; returns the per-snapshot mean loss."""
+        tc = TrainConfig(int(snapshots), int(iters_per_snapshot), int(batch_chunks), float(dt),
+                         float(lr), int(seed))
+        curve = np.zeros(int(snapshots), np.float32)
+        check(self.lib.sphk_surrogate_train(self.h, C.byref(tc), _ptr(curve, C.c_float),
+                                            curve.size))
+        return curve
+
+    def surrogate_macro_step(self, dt: float):
+        check(self.lib.sphk_surrogate_macro_step(self.h, float(dt)))
+
+    def surrogate_run(self, macro_steps: int, dt: float) -> SurrogateReport:
+        rep = SurrogateReport()
+        check(self.lib.sphk_surrogate_run(self.h, C.c_uint64(macro_steps), float(dt),
+                                          C.byref(rep)))
+        return rep
+
+    def surrogate_densities(self) -> np.ndarray:
+        out = np.empty(self.size(), np.float32)
+        check(self.lib.sphk_surrogate_densities(self.h, _ptr(out, C.c_float), out.size))
+        return out
+
+
+def surrogate_param_count() -> int:
+    n = _SZ()
+    check(load().sphk_surrogate_param_count(C.byref(n)))
+    return n.value
+
+
+def surrogate_default_config(**overrides) -> SurrogateConfig:
+    c = SurrogateConfig()
+    check(load().sphk_surrogate_default_config(C.byref(c)))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
 
 
 def version() -> str:
