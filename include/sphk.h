@@ -191,7 +191,8 @@ const char* sphk_walk_shape_name(int shape);
  * No counterpart in the reference (its repository holds no learned component).  A
  * message-passing MLP over the step's own neighbour lists predicts the acceleration of one
  * macro-step of `substeps` plain steps:
- *   edge e_ij = [r_ij/R, (v_i - v_j)/v_s, |r_ij|/R, W(|r_ij|)/W(0)]     (8)
+ *   edge e_ij = [r_ij/R, (v_i - v_j)/v_s, |r_ij|/R, W(|r_ij|)/W(0),
+ *                (rho_j - rho0) k_rho]                                      (9)
  *   S_i = c sum_{j != i} ReLU(W1 e_ij + b1)      m_i = W2 S_i + c n_i b2     (128, 128)
  *   z_i = ReLU(Wn1 [m_i, (rho_i - rho0) k_rho, v_i / v_s] + bn1)           (132 -> 128)
  *   a_i = a_s (Wn2 z_i + bn2)                                                (3)
@@ -237,13 +238,13 @@ typedef struct {
 } sphk_surrogate_report;
 
 int sphk_surrogate_default_config(sphk_surrogate_config* out);
-/* Number of FP32 parameters (35,075). */
+/* Number of FP32 parameters (35,203). */
 int sphk_surrogate_param_count(size_t* out);
 /* Creates (or resets) the engine's network: Xavier-uniform weights from `seed`, zero
  * biases, zero Adam state.  NULL cfg = defaults. */
 int sphk_surrogate_init(sphk_engine* e, const sphk_surrogate_config* cfg, uint64_t seed);
 int sphk_surrogate_get_config(sphk_engine* e, sphk_surrogate_config* out);
-/* Flat parameters W1[128][8] b1[128] W2[128][128] b2[128] Wn1[128][132] bn1[128]
+/* Flat parameters W1[128][9] b1[128] W2[128][128] b2[128] Wn1[128][132] bn1[128]
  * Wn2[3][128] bn2[3].  set_params resets the Adam state. */
 int sphk_surrogate_get_params(sphk_engine* e, float* out, size_t count);
 int sphk_surrogate_set_params(sphk_engine* e, const float* params, size_t count);

@@ -118,6 +118,7 @@ size_t    or_brute_neighbors(size_t n, const float* pos, float r2, int64_t* offs
 int       or_num_threads(void);
 
 /* ---- learned macro-step surrogate (surrogate_oracle.c) ---- */
+#define OR_SUR_NF 9 /* edge features */
 typedef struct {
     int32_t h1, h2, h3; /* widths: message hidden, message output, node hidden */
     int32_t bf16;       /* 1: round to bfloat16 where the device does */

@@ -9,7 +9,8 @@
  *   edge features of target i and neighbour j (j != i, d^2 <= R^2, the reference's
  *   neighbour set of src/spatial_hash.cpp:31-57), all FP32:
  *     r = x_i - x_j, d2 = (rx^2 + ry^2) + rz^2, |r| = sqrt(d2), q = |r| / h
- *     e = [r * inv_r (3), (v_i - v_j) * inv_v (3), |r| * inv_r, w_shape(q) * w_norm]
+ *     e = [r * inv_r (3), (v_i - v_j) * inv_v (3), |r| * inv_r, w_shape(q) * w_norm,
+ *          (rho_j - rho0) * rho_scale]
  *   message:  u_ij = W1 e_ij + b1 (h1),  S_i = msg_scale * sum_j ReLU(u_ij)
  *             m_i  = W2 S_i + msg_scale * n_i * b2 (h2; n_i = number of edges)
  *   node:     x_i = [m_i, (rho_i - rho0) * rho_scale, v_i * inv_v]  (h2 + 4)
@@ -19,7 +20,7 @@
  *
  * The second message layer is linear, so sum_j (W2 h_ij + b2) = W2 sum_j h_ij + n_i b2
  * and it is applied once per particle.  Parameters are one flat FP32 array:
- *   W1[h1][8] b1[h1] W2[h2][h1] b2[h2] Wn1[h3][h2+4] bn1[h3] Wn2[3][h3] bn2[3]
+ *   W1[h1][9] b1[h1] W2[h2][h1] b2[h2] Wn1[h3][h2+4] bn1[h3] Wn2[3][h3] bn2[3]
  * (row-major, output index first).  With cfg.bf16 set, values are rounded to bfloat16
  * (round to nearest even) where the device rounds them: every parameter, the edge
  * features, S_i, the node input x_i and z_i.  All other arithmetic is FP64.
@@ -54,7 +55,7 @@ float or_bf16_round(float f) {
 
 size_t or_sur_nparams(const or_sur_cfg* c) {
     const size_t h1 = (size_t)c->h1, h2 = (size_t)c->h2, h3 = (size_t)c->h3;
-    return h1 * 8 + h1 + h2 * h1 + h2 + h3 * (h2 + 4) + h3 + 3 * h3 + 3;
+    return h1 * OR_SUR_NF + h1 + h2 * h1 + h2 + h3 * (h2 + 4) + h3 + 3 * h3 + 3;
 }
 
 typedef struct {
@@ -65,7 +66,7 @@ static offs_t offsets(const or_sur_cfg* c) {
     const size_t h1 = (size_t)c->h1, h2 = (size_t)c->h2, h3 = (size_t)c->h3;
     offs_t o;
     o.W1 = 0;
-    o.b1 = o.W1 + h1 * 8;
+    o.b1 = o.W1 + h1 * OR_SUR_NF;
     o.W2 = o.b1 + h1;
     o.b2 = o.W2 + h2 * h1;
     o.Wn1 = o.b2 + h2;
@@ -93,15 +94,15 @@ static float w_shape(float q) {
 }
 
 /* Edge features of (i, j) in FP32, rounded per cfg. */
-static void edge_features(const or_sur_cfg* c, const float* pos, const float* vel, int64_t i,
-                          int64_t j, double e[8]) {
+static void edge_features(const or_sur_cfg* c, const float* pos, const float* vel,
+                          const float* rho, int64_t i, int64_t j, double e[OR_SUR_NF]) {
     const float rx = pos[3 * i] - pos[3 * j];
     const float ry = pos[3 * i + 1] - pos[3 * j + 1];
     const float rz = pos[3 * i + 2] - pos[3 * j + 2];
     const float d2 = (rx * rx + ry * ry) + rz * rz;
     const float r = sqrtf(d2);
     const float q = r * c->inv_h;
-    float f[8];
+    float f[OR_SUR_NF];
     f[0] = rx * c->inv_r;
     f[1] = ry * c->inv_r;
     f[2] = rz * c->inv_r;
@@ -110,7 +111,8 @@ static void edge_features(const or_sur_cfg* c, const float* pos, const float* ve
     f[5] = (vel[3 * i + 2] - vel[3 * j + 2]) * c->inv_v;
     f[6] = r * c->inv_r;
     f[7] = w_shape(q) * c->w_norm;
-    for (int k = 0; k < 8; ++k) e[k] = rnd(c, (double)f[k]);
+    f[8] = (rho[j] - c->rho0) * c->rho_scale;
+    for (int k = 0; k < OR_SUR_NF; ++k) e[k] = rnd(c, (double)f[k]);
 }
 
 /* Parameters as the device sees them. */
@@ -135,16 +137,16 @@ static void node_forward(const or_sur_cfg* c, const double* p, const offs_t* o,
                          const float* pos, const float* vel, const float* rho,
                          const int64_t* off, const int64_t* idx, int64_t i, node_t* nd) {
     const int h1 = c->h1, h2 = c->h2, h3 = c->h3;
-    double e[8];
+    double e[OR_SUR_NF];
     for (int a = 0; a < h1; ++a) nd->S[a] = 0.0;
     nd->n = 0;
     for (int64_t t = off[i]; t < off[i + 1]; ++t) {
         const int64_t j = idx[t];
         if (j == i) continue;
-        edge_features(c, pos, vel, i, j, e);
+        edge_features(c, pos, vel, rho, i, j, e);
         for (int a = 0; a < h1; ++a) {
             double u = p[o->b1 + a];
-            for (int k = 0; k < 8; ++k) u += p[o->W1 + (size_t)a * 8 + k] * e[k];
+            for (int k = 0; k < OR_SUR_NF; ++k) u += p[o->W1 + (size_t)a * OR_SUR_NF + k] * e[k];
             if (u > 0.0) nd->S[a] += u;
         }
         nd->n++;
@@ -233,7 +235,7 @@ double or_sur_loss_grad(const or_sur_cfg* c, const float* params, size_t n, cons
         double* dz = (double*)malloc((size_t)h3 * sizeof(double));
         double* dm = (double*)malloc((size_t)h2 * sizeof(double));
         double* dS = (double*)malloc((size_t)h1 * sizeof(double));
-        double e[8];
+        double e[OR_SUR_NF];
 #pragma omp for schedule(dynamic, 16)
         for (int64_t i = 0; i < (int64_t)n; ++i) {
             if (!in_batch[i]) continue;
@@ -271,13 +273,15 @@ double or_sur_loss_grad(const or_sur_cfg* c, const float* params, size_t n, cons
             for (int64_t t = off[i]; t < off[i + 1]; ++t) {
                 const int64_t j = idx[t];
                 if (j == i) continue;
-                edge_features(c, pos, vel, i, j, e);
+                edge_features(c, pos, vel, rho, i, j, e);
                 for (int a = 0; a < h1; ++a) {
                     double u = p[o.b1 + a];
-                    for (int k = 0; k < 8; ++k) u += p[o.W1 + (size_t)a * 8 + k] * e[k];
+                    for (int k = 0; k < OR_SUR_NF; ++k)
+                        u += p[o.W1 + (size_t)a * OR_SUR_NF + k] * e[k];
                     if (u > 0.0) {
                         g[o.b1 + a] += dS[a];
-                        for (int k = 0; k < 8; ++k) g[o.W1 + (size_t)a * 8 + k] += dS[a] * e[k];
+                        for (int k = 0; k < OR_SUR_NF; ++k)
+                            g[o.W1 + (size_t)a * OR_SUR_NF + k] += dS[a] * e[k];
                     }
                 }
             }

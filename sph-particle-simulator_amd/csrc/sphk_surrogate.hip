@@ -51,14 +51,14 @@ __device__ __forceinline__ float w_shape_ieee(float q) {
     return q <= 1.0f ? win : (q <= 2.0f ? wout : 0.0f);
 }
 
-// Edge features of one pair (zero when `edge` is false), rounded to bf16.
-__device__ __forceinline__ bf16x8 edge_features(bool edge, float4 pi, float4 vi, float4 pj,
-                                                float4 vj, float d2, const SurScales& sc) {
+// Edge features of one pair in FP32, zero when `edge` is false.
+__device__ __forceinline__ void edge_values(bool edge, float4 pi, float4 vi, float4 pj,
+                                            float4 vj, float d2, const SurScales& sc,
+                                            float (&f)[9]) {
 #pragma clang fp contract(off)
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float rl = __builtin_amdgcn_sqrtf(d2);
     const float w = w_shape_ieee(rl * sc.inv_h);
-    float f[8];
     f[0] = rx * sc.inv_r;
     f[1] = ry * sc.inv_r;
     f[2] = rz * sc.inv_r;
@@ -67,15 +67,30 @@ __device__ __forceinline__ bf16x8 edge_features(bool edge, float4 pi, float4 vi,
     f[5] = (vi.z - vj.z) * sc.inv_v;
     f[6] = rl * sc.inv_r;
     f[7] = w * sc.w_norm;
-    bf16x8 e;
+    f[8] = (vj.w - sc.rho0) * sc.rho_scale;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) e[k] = (__bf16)(edge ? f[k] : 0.0f);
-    return e;
+    for (int k = 0; k < 9; ++k) f[k] = edge ? f[k] : 0.0f;
+}
+
+// Edge features of one pair rounded to bf16: features 0..7 as one operand vector, feature 8
+// (neighbour density) apart.
+__device__ __forceinline__ void edge_features(bool edge, float4 pi, float4 vi, float4 pj,
+                                              float4 vj, float d2, const SurScales& sc,
+                                              bf16x8& e, __bf16& e8) {
+    float f[9];
+    edge_values(edge, pi, vi, pj, vj, d2, sc, f);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) e[k] = (__bf16)f[k];
+    e8 = (__bf16)f[8];
+}
+
+__device__ __forceinline__ uint32_t bits16(__bf16 x) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, x);
 }
 
 // B operands of message layer 1 for hidden tile hb (n = 32 hb + l % 32).  The K range of an
-// MFMA is 16: one half of the lanes supplies the 8 edge features, the other half a bias
-// slot (k = 8 for block 0, k = 0 for block 1) so that W1 e + b1 is one product.
+// MFMA is 16: one half of the lanes supplies W1 columns 0..7, the other half [b1, W1 column 8]
+// (k = 8, 9 for block 0; k = 0, 1 for block 1), so W1 e + b1 is one product.
 struct Layer1B {
     bf16x8 blk0[4], blk1[4];
 };
@@ -90,20 +105,23 @@ __device__ __forceinline__ void load_layer1(const __bf16* __restrict__ pk, uint3
         const bf16x8 w = load8(pk + kPkW1 + n * 8u);
         bf16x8 b = zero8();
         b[0] = pk[kPkB1 + n];
+        b[1] = pk[kPkW1r + n];
         B.blk0[hb] = hi ? b : w;
         B.blk1[hb] = hi ? w : b;
     }
 }
 
-// A operands: row r of block b is the edge of lane 32 b + r.  The lane that owns the edge
-// supplies its features; the partner lane (l ^ 32) supplies the bias slot (1 for a real
-// edge, 0 for padding, so padding rows are exactly zero).
-__device__ __forceinline__ void layer1_a(bf16x8 fe, bool edge, uint32_t lane, bf16x8& a0,
-                                         bf16x8& a1) {
+// A operands: row r of block b is the edge of lane 32 b + r.  Its owner supplies features
+// 0..7; the partner lane (l ^ 32) supplies [1, feature 8] (zeros for a padding row, which is
+// then exactly zero).
+__device__ __forceinline__ void layer1_a(bf16x8 fe, __bf16 e8, bool edge, uint32_t lane,
+                                         bf16x8& a0, bf16x8& a1) {
     const unsigned long long bal = __ballot(edge);
     const bool pv = ((bal >> (lane ^ 32u)) & 1ull) != 0ull;
+    const uint32_t e8p = (uint32_t)__shfl_xor((int)bits16(e8), 32);
     bf16x8 other = zero8();
     other[0] = (__bf16)(pv ? 1.0f : 0.0f);
+    other[1] = __builtin_bit_cast(__bf16, (uint16_t)e8p);
     const bool hi = lane >= 32u;
     a0 = hi ? other : fe;
     a1 = hi ? fe : other;
@@ -119,66 +137,200 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t x) {
     return uni(x);
 }
 
-// Walks every neighbour j != i of each lane's particle in uniform steps over the wave and
-// calls body(edge, pj, vj, d2) once per step; lanes with nothing left get edge = false.
-// The neighbour words come from the density pass (re-tested for chunks beyond kMaxWords).
-template <int kSeg, class Body>
-__device__ __forceinline__ void for_each_edge(uint32_t item, const Chunk& ch, uint32_t s,
-                                              bool active, float4 pi,
-                                              const WalkBuffers& wb, const float4* spm,
-                                              const float4* svr, const uint32_t* cs,
-                                              const StepConsts& c, uint32_t* lmask,
-                                              uint32_t* lwj0, uint32_t lane, Body&& body) {
+constexpr int kEdgeWaves = 2;   // one chunk per workgroup: 32 targets per wavefront
+constexpr int kCols = 32;       // target columns of a wavefront
+constexpr int kListCap = 72;    // non-empty words per target column held at once
+constexpr int kStage = 136;     // LDS row stride (bf16) of staged 128-wide rows
+constexpr int kStageX = 152;    // LDS row stride (bf16) of staged node-input rows
+constexpr int kBatch = 16;      // stored words requested per batch while building a list
+
+// LDS of one wavefront's walk: per target column, its non-empty words (mask, first slot)
+// in candidate order, entry i of column c at list[i * 32 + c].
+struct alignas(16) WalkLds {
+    uint2 list[kListCap * kCols];
+};
+
+// Appends the words of a window to the column lists, up to kListCap non-empty words per
+// column.  Stored words are read from the density pass; otherwise candidates are tested here
+// (phase 1 with the contracted filter).  Returns false when the chunk has no words left.
+struct ListBuilder {
+    int wpos = 0;        // next stored word
+    uint32_t n_ne = 0;   // non-empty words of this lane's column in the window
+    uint32_t cnt = 0;    // neighbour bits of this lane's column in the window
+};
+
+__device__ __forceinline__ bool build_list(ListBuilder& lb, bool stored, int total_words,
+                                           const WalkBuffers& wb, uint32_t item,
+                                           uint32_t clane, Cursor& cu, const Chunk& ch,
+                                           const float4* spm, const uint32_t* cs_,
+                                           const StepConsts& c, float4 pi, uint32_t lane,
+                                           WalkLds& L) {
+    const uint32_t col = lane & 31u;
+    const bool writer = lane < 32u;
+    lb.n_ne = 0u;
+    lb.cnt = 0u;
+    bool any = false;
+    if (stored) {
+        const size_t base = (size_t)item * kMaxWords;
+        // a window of up to kListCap words cannot overflow a column list
+        const int end = min(total_words, lb.wpos + kListCap);
+        while (lb.wpos < end) {
+            const int nb = min(kBatch, end - lb.wpos);
+            uint32_t wv[kBatch];
+            uint32_t jv[kBatch];
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                const int w = lb.wpos + min(k, nb - 1);
+                wv[k] = wb.mask[(base + w) * 64 + clane];
+                jv[k] = uni(wb.wj0[base + w]);
+            }
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                if (k < nb && wv[k] != 0u) {
+                    if (writer) L.list[lb.n_ne * kCols + col] = make_uint2(wv[k], jv[k]);
+                    ++lb.n_ne;
+                    lb.cnt += (uint32_t)__builtin_popcount(wv[k]);
+                }
+            }
+            lb.wpos += nb;
+            any = true;
+        }
+    } else {
+        while (__ballot(lb.n_ne >= (uint32_t)kListCap) == 0ull && cursor_settle(cu, ch, cs_, c)) {
+            const uint32_t j0 = uni(cu.j);
+            const uint32_t cnt = uni(min(32u, cu.end - j0));
+            uint32_t word = 0;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4 q = spm[j0 + k];
+                word = push_bit(word, dist2_filter(pi.x, pi.y, pi.z, q.x, q.y, q.z), c.r2_filter);
+            }
+            word <<= (32u - cnt) & 31u;  // left-aligned: candidate k in bit 31 - k
+            if (word != 0u) {
+                if (writer) L.list[lb.n_ne * kCols + col] = make_uint2(word, j0);
+                ++lb.n_ne;
+                lb.cnt += (uint32_t)__builtin_popcount(word);
+            }
+            cu.j = j0 + cnt;
+            any = true;
+        }
+    }
+    return any;
+}
+
+// Walk of one lane over its target's list: the lower lane of a target (p = 0) takes bits
+// from the front, the upper one (p = 1) from the back, each its share of the count, so the
+// two never meet.  `e` is the list entry at `idx`, read one step ahead.
+struct PairWalk {
+    uint32_t m = 0u, jb = 0u;
+    int idx = 0, dir = 1;
+    uint2 e;
+    __device__ __forceinline__ void begin(uint32_t p, uint32_t n_ne) {
+        m = 0u;
+        jb = 0u;
+        dir = p ? -1 : 1;
+        idx = p ? (int)n_ne - 1 : 0;
+    }
+    __device__ __forceinline__ void request(const WalkLds& L, uint32_t col) {
+        e = L.list[(uint32_t)max(idx, 0) * kCols + col];
+    }
+    __device__ __forceinline__ uint32_t next(uint32_t p) {
+        const bool refill = m == 0u;
+        m = refill ? e.x : m;
+        jb = refill ? e.y : jb;
+        idx += refill ? dir : 0;
+        const uint32_t bit = p ? (uint32_t)__builtin_ctz(m) : 31u - (uint32_t)__builtin_clz(m);
+        m ^= 1u << bit;
+        return jb + (31u - bit);
+    }
+};
+
+// Walks every neighbour j != i of the wavefront's 32 targets, two lanes per target, in
+// uniform steps; body(edge, pj, vj, d2) is called once per step (edge = false on lanes with
+// nothing left).  Neighbour data is requested one step ahead.
+template <class Body>
+__device__ __forceinline__ void walk_pairs(uint32_t item, const Chunk& ch, uint32_t s,
+                                           bool active, float4 pi, uint32_t clane,
+                                           const WalkBuffers& wb, const float4* spm,
+                                           const float4* svr, const uint32_t* cs_,
+                                           const StepConsts& c, WalkLds& L, uint32_t lane,
+                                           Body&& body) {
+    const uint32_t col = lane & 31u, p = lane >> 5;
     const int total_words = (int)uni(wb.nwords[item]);
     const bool stored = total_words <= kMaxWords;
     const uint32_t s_safe = active ? s : ch.s0;
     Cursor cu;
-    if (!stored) cursor_begin(cu, ch, cs, c);
-    int seg_base = 0;
+    if (!stored) cursor_begin(cu, ch, cs_, c);
+    ListBuilder lb;
     for (;;) {
-        int nw;
-        SegLane sl;
-        if (stored) {
-            nw = min(kSeg, total_words - seg_base);
-            if (nw <= 0) break;
-            const size_t base = (size_t)item * kMaxWords + seg_base;
-            sl.nonempty = 0;
-            sl.count = 0;
-            for (int w = 0; w < nw; ++w) {
-                const uint32_t word = wb.mask[(base + w) * 64 + lane];
-                lmask[w * 64 + lane] = word;
-                sl.nonempty |= (word != 0u ? 1u : 0u) << w;
-                sl.count += (uint32_t)__builtin_popcount(word);
-            }
-            for (int w = (int)lane; w < nw; w += 64) lwj0[w] = wb.wj0[base + w];
-        } else {
-            nw = phase1_segment<kSeg>(cu, ch, spm, cs, c, pi.x, pi.y, pi.z, lmask, lwj0, lane,
-                                      sl);
-            if (nw == 0) break;
-        }
+        if (!build_list(lb, stored, total_words, wb, item, clane, cu, ch, spm, cs_, c, pi, lane,
+                        L))
+            break;
         wave_sync();
-        const uint32_t tmax = wave_max(sl.count);
-        BitWalk<kSeg> bw;
-        bw.ne = sl.nonempty;
+        const uint32_t quota = p ? lb.cnt >> 1 : (lb.cnt + 1u) >> 1;
+        const uint32_t tmax = wave_max(quota);
+        PairWalk pw;
+        pw.begin(p, lb.n_ne);
+        pw.request(L, col);
+        uint32_t jn = s_safe;
+        if (quota > 0u) jn = pw.next(p);
+        pw.request(L, col);
+        float4 pn = spm[jn];
+        float4 vn = svr[jn];
         for (uint32_t t = 0; t < tmax; ++t) {
-            const bool more = t < sl.count;
-            uint32_t j = s_safe;
-            if (more) j = bw.next(lmask, lwj0, lane);
-            const float4 pj = spm[j];
-            const float4 vj = svr[j];
+            const uint32_t j = jn;
+            const bool has = t < quota;
+            const float4 pj = pn;
+            const float4 vj = vn;
+            jn = s_safe;
+            if (t + 1u < quota) jn = pw.next(p);
+            pw.request(L, col);
+            pn = spm[jn];
+            vn = svr[jn];
             const float d2 = dist2_exact(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
-            const bool edge = more && j != s && d2 <= c.r2;  // the reference's test, self excluded
+            const bool edge = has && j != s && d2 <= c.r2;  // the reference's test, self excluded
             body(edge, pj, vj, d2);
         }
-        seg_base += nw;
         wave_sync();
-        if (!stored && cu.done) break;
     }
 }
 
-constexpr int kEdgeSeg = 8;
-constexpr int kEdgeWaves = 2;
-constexpr int kStage = 136;  // LDS row stride (bf16) of the staged S rows
+// One macro-step of a particle: v' = d^k v + a T, x' = x + T (v + v') / 2, then the wall
+// clamp of the plain step (src/particle.cpp:122-153).
+__device__ __forceinline__ void macro_integrate(float4& p, float3& v1, float3 v0, float3 a,
+                                                float T, float dk, const SceneGrid& g) {
+#pragma clang fp contract(off)
+    v1 = make_float3(dk * v0.x + a.x * T, dk * v0.y + a.y * T, dk * v0.z + a.z * T);
+    p.x = p.x + (0.5f * T) * (v0.x + v1.x);
+    p.y = p.y + (0.5f * T) * (v0.y + v1.y);
+    p.z = p.z + (0.5f * T) * (v0.z + v1.z);
+    const float wd = -0.8f;
+    if (p.x < g.lo[0]) { p.x = g.lo[0]; v1.x = v1.x * wd; } else if (p.x > g.hi[0]) { p.x = g.hi[0]; v1.x = v1.x * wd; }
+    if (p.y < g.lo[1]) { p.y = g.lo[1]; v1.y = v1.y * wd; } else if (p.y > g.hi[1]) { p.y = g.hi[1]; v1.y = v1.y * wd; }
+    if (p.z < g.lo[2]) { p.z = g.lo[2]; v1.z = v1.z * wd; } else if (p.z > g.hi[2]) { p.z = g.hi[2]; v1.z = v1.z * wd; }
+}
+
+// Row-tile GEMM of one wavefront: D[t][n] (+)= sum_k A[t][k] Bt[n][k] for the 32 target rows
+// of A (LDS, row stride lda) and the 4 output tiles of Bt (global, row stride ldb), K = 16 ks.
+__device__ __forceinline__ void tile_gemm(f32x16 (&d)[4], const __bf16* A, int lda,
+                                          const __bf16* __restrict__ Bt, int ldb, int ks,
+                                          int ntiles, uint32_t lane) {
+    const uint32_t col = lane & 31u, h = lane >> 5;
+    for (int k = 0; k < ks; ++k) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(A + col * lda + 16 * k + 8 * h);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (t < ntiles)
+                d[t] = mfma16(a, load8(Bt + (size_t)(32 * t + col) * ldb + 16 * k + 8 * h), d[t]);
+    }
+}
+
+// Activation staging of one wavefront (reuses the list once the walk is done).
+struct EpiLds {
+    __bf16 s[32 * kStage];      // S, later z (row stride kStage)
+    __bf16 x[32 * kStageX];     // node input (row stride kStageX)
+};
+
+static_assert(sizeof(EpiLds) <= sizeof(WalkLds), "activation staging reuses the list");
 
 }  // namespace
 
@@ -192,9 +344,11 @@ __global__ __launch_bounds__(256) void k_sur_pack(const float* __restrict__ p,
     if (i < kPkTotal) {
         float v;
         if (i < kPkB1) {
-            v = p[kOffW1 + i];
-        } else if (i < kPkW2) {
+            v = p[kOffW1 + (i / 8u) * kSurNF + i % 8u];
+        } else if (i < kPkW1r) {
             v = p[kOffB1 + (i - kPkB1)];
+        } else if (i < kPkW2) {
+            v = p[kOffW1 + (i - kPkW1r) * kSurNF + 8u];
         } else if (i < kPkW2T) {
             v = p[kOffW2 + (i - kPkW2)];
         } else if (i < kPkWn1) {
@@ -227,149 +381,229 @@ __global__ __launch_bounds__(256) void k_sur_pack(const float* __restrict__ p,
 }
 
 // ---------------------------------------------------------------------------
-// edge pass (forward): S_i = c * sum_j ReLU(W1 e_ij + b1) through
-//   sum ReLU(u) = (sum u + sum |u|) / 2,  sum_j u_ij = W1 (sum_j e_ij) + n_i b1,
-// so the per-edge work after the MFMA is one add with an |.| modifier per value.
+// forward pass.  Per chunk (two wavefronts of 32 targets):
+//   edge walk: S_i = c * sum_j ReLU(W1 e_ij + b1) through
+//       sum ReLU(u) = (sum u + sum |u|) / 2,  sum_j u_ij = W1 (sum_j e_ij) + n_i b1,
+//     so the per-edge work after the MFMA is one add with an |.| modifier per value.  The
+//     two MFMA blocks of a step hold the two lanes' edges of the same 32 targets.
+//   node MLP on MFMA from LDS: m = W2 S + c n b2, z = ReLU(Wn1 x + bn1), out = Wn2 z + bn2.
+// kIntegrate: the macro-step is applied and written to the caller-order state.  Otherwise
+// the S, X, Z and OUT rows are written for the backward pass (batch: rows 64 b + lane;
+// no batch: rows = sorted slots).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64 * kEdgeWaves) void k_sur_edges(
-    WalkBuffers wb, const uint32_t* __restrict__ batch, uint32_t nbatch,
-    const float4* __restrict__ spm, const float4* __restrict__ svr,
-    const uint32_t* __restrict__ skey, const uint32_t* __restrict__ cs, StepConsts c,
-    const SceneGrid* __restrict__ scenes, const __bf16* __restrict__ pk,
-    const float* __restrict__ pf, SurScales sc, __bf16* __restrict__ S, __bf16* __restrict__ X,
-    __bf16* __restrict__ Z, float* __restrict__ cnt, uint32_t* __restrict__ rowslot) {
-    __shared__ uint32_t lmask_all[kEdgeWaves][kEdgeSeg * 64];
-    __shared__ uint32_t lwj0_all[kEdgeWaves][kEdgeSeg];
-    __shared__ float lse_all[kEdgeWaves][64][9];
-    __shared__ __bf16 stage_all[kEdgeWaves][64 * kStage];
-    const uint32_t wave = uni(threadIdx.x >> 6);
+template <bool kIntegrate>
+__global__ __launch_bounds__(64 * kEdgeWaves) __attribute__((amdgpu_waves_per_eu(2))) void
+k_sur_forward(WalkBuffers wb, const uint32_t* __restrict__ batch, uint32_t nbatch,
+              const float4* __restrict__ spm, const float4* __restrict__ svr,
+              const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sid,
+              const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
+              const __bf16* __restrict__ pk, const float* __restrict__ pf, SurScales sc,
+              SurRows rows, SurStep step) {
+    __shared__ WalkLds lds_all[kEdgeWaves];
+    __shared__ float lout_all[kEdgeWaves][32][4];
+    __shared__ float lcnt_all[kEdgeWaves][32];
+    const uint32_t half = uni(threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
-    uint32_t* lmask = lmask_all[wave];
-    uint32_t* lwj0 = lwj0_all[wave];
-    const uint32_t w_idx = swizzled_block() * kEdgeWaves + wave;
+    const uint32_t col = lane & 31u, h = lane >> 5;
+    WalkLds& L = lds_all[half];
+    const uint32_t b_idx = swizzled_block();
     uint32_t item;
     if (batch) {
-        if (w_idx >= nbatch) return;
-        item = uni(batch[w_idx]);
+        if (b_idx >= nbatch) return;
+        item = uni(batch[b_idx]);
     } else {
-        item = w_idx;
+        item = b_idx;
     }
+    const uint32_t clane = 32u * half + col;  // this lane's target within the chunk
     Chunk ch;
     uint32_t s;
     bool active;
     float4 pi;
-    if (!load_chunk(item, wb.items, wb.nitems, skey, cs, spm, c, scenes, lane, ch, s, active,
+    if (!load_chunk(item, wb.items, wb.nitems, skey, cs, spm, c, scenes, clane, ch, s, active,
                     pi))
         return;
     const float4 vi = active ? svr[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     Layer1B B;
     load_layer1(pk, lane, B);
-    f32x16 acc0[4], acc1[4];
+    f32x16 acc[4];
 #pragma unroll
-    for (int hb = 0; hb < 4; ++hb) {
-        acc0[hb] = zero16();
-        acc1[hb] = zero16();
-    }
-    float se[8];
+    for (int hb = 0; hb < 4; ++hb) acc[hb] = zero16();
+    float se[9];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) se[k] = 0.0f;
+    for (int k = 0; k < 9; ++k) se[k] = 0.0f;
     float ne = 0.0f;
-    for_each_edge<kEdgeSeg>(
-        item, ch, s, active, pi, wb, spm, svr, cs, c, lmask, lwj0, lane,
-        [&](bool edge, float4 pj, float4 vj, float d2) {
-            const bf16x8 fe = edge_features(edge, pi, vi, pj, vj, d2, sc);
+    walk_pairs(item, ch, s, active, pi, clane, wb, spm, svr, cs, c, L, lane,
+               [&](bool edge, float4 pj, float4 vj, float d2) {
+                   bf16x8 fe;
+                   __bf16 e8;
+                   edge_features(edge, pi, vi, pj, vj, d2, sc, fe, e8);
+                   // the sums must see the values the MFMA sees (sum u is subtracted from
+                   // sum |u| for hidden units that are mostly inactive)
 #pragma unroll
-            for (int k = 0; k < 8; ++k) se[k] += (float)fe[k];
-            ne += edge ? 1.0f : 0.0f;
-            bf16x8 a0, a1;
-            layer1_a(fe, edge, lane, a0, a1);
+                   for (int k = 0; k < 8; ++k) se[k] += (float)fe[k];
+                   se[8] += (float)e8;
+                   ne += edge ? 1.0f : 0.0f;
+                   bf16x8 a0, a1;
+                   layer1_a(fe, e8, edge, lane, a0, a1);
 #pragma unroll
-            for (int hb = 0; hb < 4; ++hb) {
-                const f32x16 d0 = mfma16(a0, B.blk0[hb], zero16());
-                const f32x16 d1 = mfma16(a1, B.blk1[hb], zero16());
+                   for (int hb = 0; hb < 4; ++hb) {
+                       const f32x16 d0 = mfma16(a0, B.blk0[hb], zero16());
+                       const f32x16 d1 = mfma16(a1, B.blk1[hb], zero16());
 #pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    acc0[hb][v] += fabsf(d0[v]);
-                    acc1[hb][v] += fabsf(d1[v]);
-                }
+                       for (int v = 0; v < 16; ++v) acc[hb][v] += fabsf(d0[v]) + fabsf(d1[v]);
+                   }
+               });
+    // both lanes of a target hold half of its sums
+#pragma unroll
+    for (int k = 0; k < 9; ++k) se[k] += __shfl_xor(se[k], 32);
+    ne += __shfl_xor(ne, 32);
+    float* lcnt = lcnt_all[half];
+    if (h == 0u) lcnt[col] = ne;
+    // sum u + sum |u| = [sum e, n] W1aug^T + acc, on MFMA with the accumulators as C.  The
+    // sums enter as three bf16 parts (x = x0 + x1 + x2 to about 2^-27 relative): sum u is
+    // cancelled against sum |u| for hidden units that are mostly inactive.
+    {
+        // lanes 0..31 supply k = 0..7 (feature sums 0..7), lanes 32..63 k = 8, 9 (the edge
+        // count, which meets b1, and feature sum 8, which meets W1 column 8)
+        float x[8];
+        if (h == 0u) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) x[k] = se[k];
+        } else {
+            x[0] = ne;
+            x[1] = se[8];
+#pragma unroll
+            for (int k = 2; k < 8; ++k) x[k] = 0.0f;
+        }
+#pragma unroll
+        for (int part = 0; part < 3; ++part) {
+            bf16x8 a;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                a[k] = (__bf16)x[k];
+                x[k] = x[k] - (float)a[k];
             }
-        });
-    // per-target feature sums -> LDS, then S = c (sum u + sum |u|) / 2 staged as bf16 rows
-    float(*lse)[9] = lse_all[wave];
-    __bf16* stage = stage_all[wave];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) lse[lane][k] = se[k];
-    lse[lane][8] = ne;
-    wave_sync();
-    const uint32_t col = lane & 31u, h = lane >> 5;
-    float w1f[4][8], b1f[4];
-#pragma unroll
-    for (int hb = 0; hb < 4; ++hb) {
-        const uint32_t n = 32u * hb + col;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) w1f[hb][k] = pf[kFW1 + n * 8u + k];
-        b1f[hb] = pf[kFB1 + n];
+            for (int hb = 0; hb < 4; ++hb) acc[hb] = mfma16(a, B.blk0[hb], acc[hb]);
+        }
     }
+    wave_sync();  // the list is free from here on
+    EpiLds& E = *reinterpret_cast<EpiLds*>(&L);
     const float half_c = 0.5f * sc.msg_scale;
 #pragma unroll
-    for (int blk = 0; blk < 2; ++blk) {
+    for (int hb = 0; hb < 4; ++hb)
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+            E.s[acc_row(v, h) * kStage + 32u * hb + col] = (__bf16)(half_c * acc[hb][v]);
+    // node input tail: [rho, v, 1, 0 ...] (column 132 = 1 is the bias column of dbn1)
+    if (h == 0u) {
+#pragma clang fp contract(off)
+        const bool a = active;
+        __bf16* xr = E.x + col * kStageX + 128;
+        xr[0] = (__bf16)(a ? (vi.w - sc.rho0) * sc.rho_scale : 0.0f);
+        xr[1] = (__bf16)(a ? vi.x * sc.inv_v : 0.0f);
+        xr[2] = (__bf16)(a ? vi.y * sc.inv_v : 0.0f);
+        xr[3] = (__bf16)(a ? vi.z * sc.inv_v : 0.0f);
+        xr[4] = (__bf16)(a ? 1.0f : 0.0f);
+#pragma unroll
+        for (int k = 5; k < 16; ++k) xr[k] = (__bf16)0.0f;
+    }
+    wave_sync();
+    // m = W2 S + c n b2 -> X[:, :128]
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = zero16();
+    tile_gemm(acc, E.s, kStage, pk + kPkW2, 128, 8, 4, lane);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const uint32_t n = 32u * t + col;
+        const float b2 = pf[kFB2 + n];
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            const uint32_t tl = 32u * blk + acc_row(v, h);
-            float et[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) et[k] = lse[tl][k];
-#pragma unroll
-            for (int hb = 0; hb < 4; ++hb) {
-                float su = b1f[hb] * et[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) su += w1f[hb][k] * et[k];
-                const float a = blk == 0 ? acc0[hb][v] : acc1[hb][v];
-                stage[tl * kStage + 32u * hb + col] = (__bf16)(half_c * (su + a));
-            }
+            const uint32_t r = acc_row(v, h);
+            E.x[r * kStageX + n] = (__bf16)(acc[t][v] + sc.msg_scale * lcnt[r] * b2);
         }
     }
     wave_sync();
-    // rows out: S (128 + bias column block), X and Z tails
+    // rows for the backward pass: S (and its c n column) before z overwrites it
     const uint32_t len = ch.s_end - ch.s0;
-    const uint32_t row0 = batch ? 64u * w_idx : ch.s0;
     const bool all_rows = batch != nullptr;
-    for (uint32_t idx = lane; idx < 64u * 18u; idx += 64u) {
-        const uint32_t tl = idx / 18u, part = idx % 18u;
-        if (!all_rows && tl >= len) continue;
-        const size_t row = (size_t)row0 + tl;
-        uint4 val = make_uint4(0u, 0u, 0u, 0u);
-        if (part < 16u) {
-            val = *reinterpret_cast<const uint4*>(stage + tl * kStage + part * 8u);
-        } else if (part == 16u && tl < len) {
-            const __bf16 cn = (__bf16)(sc.msg_scale * lse[tl][8]);
-            val.x = (uint32_t)__builtin_bit_cast(uint16_t, cn);
+    const size_t row0 = batch ? 64u * (size_t)b_idx : (size_t)ch.s0;
+    if constexpr (!kIntegrate) {
+        for (uint32_t idx = lane; idx < 32u * 18u; idx += 64u) {
+            const uint32_t tl = idx / 18u, part = idx % 18u;
+            const uint32_t cl = 32u * half + tl;
+            if (!all_rows && cl >= len) continue;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (part < 16u) {
+                val = *reinterpret_cast<const uint4*>(E.s + tl * kStage + part * 8u);
+            } else if (part == 16u && cl < len) {
+                val.x = bits16((__bf16)(sc.msg_scale * lcnt[tl]));
+            }
+            *reinterpret_cast<uint4*>(rows.S + (row0 + cl) * kSurRow + part * 8u) = val;
+            uint4 xv = make_uint4(0u, 0u, 0u, 0u);
+            if (cl < len) xv = *reinterpret_cast<const uint4*>(E.x + tl * kStageX + part * 8u);
+            *reinterpret_cast<uint4*>(rows.X + (row0 + cl) * kSurRow + part * 8u) = xv;
         }
-        *reinterpret_cast<uint4*>(S + row * kSurRow + part * 8u) = val;
+        wave_sync();
     }
-    if (all_rows || active) {
-        const size_t row = (size_t)row0 + lane;
-        uint4 xt0 = make_uint4(0u, 0u, 0u, 0u), xt1 = xt0, zt0 = xt0;
-        if (active) {
-#pragma clang fp contract(off)
-            const __bf16 f0 = (__bf16)((vi.w - sc.rho0) * sc.rho_scale);
-            const __bf16 f1 = (__bf16)(vi.x * sc.inv_v);
-            const __bf16 f2 = (__bf16)(vi.y * sc.inv_v);
-            const __bf16 f3 = (__bf16)(vi.z * sc.inv_v);
-            const uint32_t one = (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)1.0f);
-            xt0.x = (uint32_t)__builtin_bit_cast(uint16_t, f0) |
-                    ((uint32_t)__builtin_bit_cast(uint16_t, f1) << 16);
-            xt0.y = (uint32_t)__builtin_bit_cast(uint16_t, f2) |
-                    ((uint32_t)__builtin_bit_cast(uint16_t, f3) << 16);
-            xt0.z = one;  // column 132: bias column of Wn1
-            zt0.x = one;  // column 128 of Z: bias column of Wn2
+    // z = ReLU(Wn1 x + bn1) -> E.s
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = zero16();
+    tile_gemm(acc, E.x, kStageX, pk + kPkWn1, kSurRow, 9, 4, lane);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const uint32_t n = 32u * t + col;
+        const float bn1 = pf[kFBn1 + n];
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+            E.s[acc_row(v, h) * kStage + n] = (__bf16)fmaxf(acc[t][v] + bn1, 0.0f);
+    }
+    wave_sync();
+    // out = Wn2 z + bn2
+    acc[0] = zero16();
+    tile_gemm(acc, E.s, kStage, pk + kPkWn2, 128, 8, 1, lane);
+    float(*lout)[4] = lout_all[half];
+    if (col < 3u) {
+        const float bn2 = pf[kFBn2 + col];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) lout[acc_row(v, h)][col] = acc[0][v] + bn2;
+    }
+    wave_sync();
+    if constexpr (kIntegrate) {
+        if (h == 0u && active) {
+            const float3 a = make_float3(sc.acc_scale * lout[col][0], sc.acc_scale * lout[col][1],
+                                         sc.acc_scale * lout[col][2]);
+            float4 p = pi;
+            float3 v1;
+            const SceneGrid& g = scenes[ch.scene];
+            macro_integrate(p, v1, make_float3(vi.x, vi.y, vi.z), a, step.T, step.dk, g);
+            const uint32_t id = sid[s];
+            step.pm[id] = p;
+            step.vel[id] = make_float4(v1.x, v1.y, v1.z, 0.0f);
+            step.acc[id] = make_float4(a.x, a.y, a.z, 0.0f);
+            step.rho[id] = vi.w;
+            step.prs[id] = c.k_gas * (vi.w - c.rho0);
         }
-        *reinterpret_cast<uint4*>(X + row * kSurRow + 128) = xt0;
-        *reinterpret_cast<uint4*>(X + row * kSurRow + 136) = xt1;
-        *reinterpret_cast<uint4*>(Z + row * kSurRow + 128) = zt0;
-        *reinterpret_cast<uint4*>(Z + row * kSurRow + 136) = xt1;
-        cnt[row] = active ? sc.msg_scale * ne : 0.0f;
-        rowslot[row] = active ? s : kSurNoSlot;
+    } else {
+        // Z rows (+ bias column) and OUT rows
+        for (uint32_t idx = lane; idx < 32u * 18u; idx += 64u) {
+            const uint32_t tl = idx / 18u, part = idx % 18u;
+            const uint32_t cl = 32u * half + tl;
+            if (!all_rows && cl >= len) continue;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (cl < len) {
+                if (part < 16u) val = *reinterpret_cast<const uint4*>(E.s + tl * kStage + part * 8u);
+                else if (part == 16u) val.x = bits16((__bf16)1.0f);
+            }
+            *reinterpret_cast<uint4*>(rows.Z + (row0 + cl) * kSurRow + part * 8u) = val;
+        }
+        if (h == 0u && (all_rows || active)) {
+            const size_t row = row0 + clane;
+            rows.OUT[row] = active ? make_float4(lout[col][0], lout[col][1], lout[col][2], 0.0f)
+                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+            rows.cnt[row] = active ? sc.msg_scale * ne : 0.0f;
+            rows.rowslot[row] = active ? s : kSurNoSlot;
+        }
     }
 }
 
@@ -454,91 +688,97 @@ __global__ __launch_bounds__(64) void k_sur_wgrad(const __bf16* __restrict__ P, 
 }
 
 // ---------------------------------------------------------------------------
-// edge pass (backward): d(W1|b1) = sum_i sum_j (dS_i * [u_ij > 0]) (e_ij | 1)^T.  The
-// masked gradient is formed in the accumulator layout of the recomputed layer-1 product
-// and fed back as an A operand (K = edges); the edge features reach the B operand through
-// LDS.  One wavefront per batch chunk.
+// edge pass (backward): d(W1|b1) = sum_i sum_j (dS_i * [u_ij > 0]) (e_ij | 1)^T.  The masked
+// gradient is formed in the accumulator layout of the recomputed layer-1 product and fed
+// back as an A operand (K = edges); the edge features reach the B operand through LDS.  One
+// workgroup per batch chunk, 32 targets per wavefront.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_sur_edges_bwd(
+__global__ __launch_bounds__(64 * kEdgeWaves) void k_sur_edges_bwd(
     WalkBuffers wb, const uint32_t* __restrict__ batch, uint32_t nbatch,
     const float4* __restrict__ spm, const float4* __restrict__ svr,
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ cs, StepConsts c,
     const SceneGrid* __restrict__ scenes, const __bf16* __restrict__ pk, SurScales sc,
     const __bf16* __restrict__ DS, float* __restrict__ part) {
-    __shared__ uint32_t lmask[kEdgeSeg * 64];
-    __shared__ uint32_t lwj0[kEdgeSeg];
-    __shared__ __bf16 le[64 * 16];
-    const uint32_t lane = threadIdx.x;
+    __shared__ WalkLds lds_all[kEdgeWaves];
+    __shared__ __bf16 le_all[kEdgeWaves][64 * 16];
+    const uint32_t half = uni(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t col = lane & 31u, h = lane >> 5;
+    WalkLds& L = lds_all[half];
+    __bf16* le = le_all[half];
     const uint32_t b = blockIdx.x;
     if (b >= nbatch) return;
     const uint32_t item = uni(batch[b]);
+    const uint32_t clane = 32u * half + col;
     Chunk ch;
     uint32_t s;
     bool active;
     float4 pi;
-    if (!load_chunk(item, wb.items, wb.nitems, skey, cs, spm, c, scenes, lane, ch, s, active,
+    if (!load_chunk(item, wb.items, wb.nitems, skey, cs, spm, c, scenes, clane, ch, s, active,
                     pi))
         return;
     const float4 vi = active ? svr[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     Layer1B B;
     load_layer1(pk, lane, B);
-    const uint32_t col = lane & 31u, h = lane >> 5;
-    float ds[2][4][16];
+    float ds[4][16];
 #pragma unroll
-    for (int blk = 0; blk < 2; ++blk)
+    for (int hb = 0; hb < 4; ++hb)
 #pragma unroll
-        for (int hb = 0; hb < 4; ++hb)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const size_t row = 64u * b + 32u * blk + acc_row(v, h);
-                ds[blk][hb][v] = (float)DS[row * 128 + 32u * hb + col];
-            }
+        for (int v = 0; v < 16; ++v) {
+            const size_t row = 64u * (size_t)b + 32u * half + acc_row(v, h);
+            ds[hb][v] = (float)DS[row * 128 + 32u * hb + col];
+        }
     f32x16 accw[4];
 #pragma unroll
     for (int hb = 0; hb < 4; ++hb) accw[hb] = zero16();
-    for_each_edge<kEdgeSeg>(
-        item, ch, s, active, pi, wb, spm, svr, cs, c, lmask, lwj0, lane,
-        [&](bool edge, float4 pj, float4 vj, float d2) {
-            const bf16x8 fe = edge_features(edge, pi, vi, pj, vj, d2, sc);
-            bf16x8 ext = zero8();
-            ext[0] = (__bf16)(edge ? 1.0f : 0.0f);
-            *reinterpret_cast<bf16x8*>(le + lane * 16) = fe;
-            *reinterpret_cast<bf16x8*>(le + lane * 16 + 8) = ext;
-            wave_sync();
-            bf16x8 a[2];
-            layer1_a(fe, edge, lane, a[0], a[1]);
+    walk_pairs(item, ch, s, active, pi, clane, wb, spm, svr, cs, c, L, lane,
+               [&](bool edge, float4 pj, float4 vj, float d2) {
+                   bf16x8 fe;
+                   __bf16 e8;
+                   edge_features(edge, pi, vi, pj, vj, d2, sc, fe, e8);
+                   // E row of this lane's edge: [features 0..8, 1, 0 ...]
+                   bf16x8 ext = zero8();
+                   ext[0] = e8;
+                   ext[1] = (__bf16)(edge ? 1.0f : 0.0f);
+                   *reinterpret_cast<bf16x8*>(le + lane * 16) = fe;
+                   *reinterpret_cast<bf16x8*>(le + lane * 16 + 8) = ext;
+                   wave_sync();
+                   bf16x8 a[2];
+                   layer1_a(fe, e8, edge, lane, a[0], a[1]);
 #pragma unroll
-            for (int blk = 0; blk < 2; ++blk) {
-                bf16x8 e[2];
+                   for (int blk = 0; blk < 2; ++blk) {
+                       bf16x8 e[2];
 #pragma unroll
-                for (int g = 0; g < 2; ++g)
+                       for (int g = 0; g < 2; ++g)
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const uint32_t er = 32u * blk + 16u * g + acc_row(i, h);
-                        e[g][i] = col < 9u ? le[er * 16 + col] : (__bf16)0.0f;
-                    }
+                           for (int i = 0; i < 8; ++i) {
+                               const uint32_t er = 32u * blk + 16u * g + acc_row(i, h);
+                               e[g][i] = col < 10u ? le[er * 16 + col] : (__bf16)0.0f;
+                           }
 #pragma unroll
-                for (int hb = 0; hb < 4; ++hb) {
-                    const f32x16 d = mfma16(a[blk], blk == 0 ? B.blk0[hb] : B.blk1[hb], zero16());
-                    bf16x8 g0, g1;
+                       for (int hb = 0; hb < 4; ++hb) {
+                           const f32x16 d =
+                               mfma16(a[blk], blk == 0 ? B.blk0[hb] : B.blk1[hb], zero16());
+                           bf16x8 g0, g1;
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        g0[i] = (__bf16)(d[i] > 0.0f ? ds[blk][hb][i] : 0.0f);
-                        g1[i] = (__bf16)(d[8 + i] > 0.0f ? ds[blk][hb][8 + i] : 0.0f);
-                    }
-                    accw[hb] = mfma16(g0, e[0], accw[hb]);
-                    accw[hb] = mfma16(g1, e[1], accw[hb]);
-                }
-            }
-            wave_sync();
-        });
+                           for (int i = 0; i < 8; ++i) {
+                               g0[i] = (__bf16)(d[i] > 0.0f ? ds[hb][i] : 0.0f);
+                               g1[i] = (__bf16)(d[8 + i] > 0.0f ? ds[hb][8 + i] : 0.0f);
+                           }
+                           accw[hb] = mfma16(g0, e[0], accw[hb]);
+                           accw[hb] = mfma16(g1, e[1], accw[hb]);
+                       }
+                   }
+                   wave_sync();
+               });
     if (col < 16u) {
+        const size_t pbase = (2u * (size_t)b + half) * 128u;
 #pragma unroll
         for (int hb = 0; hb < 4; ++hb)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const uint32_t m = 32u * hb + acc_row(v, h);
-                part[((size_t)b * 128u + m) * 16u + col] = accw[hb][v];
+                part[(pbase + m) * 16u + col] = accw[hb][v];
             }
     }
 }
@@ -622,43 +862,30 @@ __global__ __launch_bounds__(256) void k_sur_targets(const float4* __restrict__ 
                             (b.z - dk * a.z) * inv_Tas, 0.0f);
 }
 
-__global__ __launch_bounds__(256) void k_sur_integrate(
-    const float* __restrict__ out, const float4* __restrict__ spm,
-    const float4* __restrict__ svr, const uint32_t* __restrict__ sid,
-    const uint32_t* __restrict__ cs, uint32_t n, StepConsts c,
-    const SceneGrid* __restrict__ scenes, float T, float dk, float acc_scale,
-    float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc,
-    float* __restrict__ rho, float* __restrict__ prs) {
-#pragma clang fp contract(off)
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+// Particles with a non-finite coordinate (dense index G) have no neighbours: free fall over
+// the macro-step, as in the plain step.
+__global__ __launch_bounds__(256) void k_sur_nonfinite(const float4* __restrict__ spm,
+                                                       const float4* __restrict__ svr,
+                                                       const uint32_t* __restrict__ sid,
+                                                       const uint32_t* __restrict__ cs,
+                                                       uint32_t n, StepConsts c,
+                                                       const SceneGrid* __restrict__ scenes,
+                                                       SurStep step) {
+    const uint32_t s = cs[c.G] + blockIdx.x * 256u + threadIdx.x;
     if (s >= n) return;
-    const uint32_t first_nonfinite = cs[c.G];
-    const float4 p = spm[s];
     const float4 vr = svr[s];
     const uint32_t id = sid[s];
-    float3 a;
-    float rho_i;
-    if (s < first_nonfinite) {
-        a = make_float3(acc_scale * out[4 * (size_t)s], acc_scale * out[4 * (size_t)s + 1],
-                        acc_scale * out[4 * (size_t)s + 2]);
-        rho_i = vr.w;
-    } else {  // no neighbours, not even itself: free fall, as in the plain step
-        a = make_float3(0.0f, c.grav, 0.0f);
-        rho_i = p.w * c.w0;
-    }
-    float3 v1 = make_float3(dk * vr.x + a.x * T, dk * vr.y + a.y * T, dk * vr.z + a.z * T);
-    float4 q = make_float4(p.x + (0.5f * T) * (vr.x + v1.x), p.y + (0.5f * T) * (vr.y + v1.y),
-                           p.z + (0.5f * T) * (vr.z + v1.z), p.w);
-    const SceneGrid& g = scenes[scene_of_particle(scenes, c.nscenes, id)];
-    const float wd = -0.8f;
-    if (q.x < g.lo[0]) { q.x = g.lo[0]; v1.x = v1.x * wd; } else if (q.x > g.hi[0]) { q.x = g.hi[0]; v1.x = v1.x * wd; }
-    if (q.y < g.lo[1]) { q.y = g.lo[1]; v1.y = v1.y * wd; } else if (q.y > g.hi[1]) { q.y = g.hi[1]; v1.y = v1.y * wd; }
-    if (q.z < g.lo[2]) { q.z = g.lo[2]; v1.z = v1.z * wd; } else if (q.z > g.hi[2]) { q.z = g.hi[2]; v1.z = v1.z * wd; }
-    pm[id] = q;
-    vel[id] = make_float4(v1.x, v1.y, v1.z, 0.0f);
-    acc[id] = make_float4(a.x, a.y, a.z, 0.0f);
-    rho[id] = rho_i;
-    prs[id] = c.k_gas * (rho_i - c.rho0);
+    float4 p = spm[s];
+    const float3 a = make_float3(0.0f, c.grav, 0.0f);
+    float3 v1;
+    macro_integrate(p, v1, make_float3(vr.x, vr.y, vr.z), a, step.T, step.dk,
+                    scenes[scene_of_particle(scenes, c.nscenes, id)]);
+    const float rho_i = p.w * c.w0;
+    step.pm[id] = p;
+    step.vel[id] = make_float4(v1.x, v1.y, v1.z, 0.0f);
+    step.acc[id] = make_float4(a.x, a.y, a.z, 0.0f);
+    step.rho[id] = rho_i;
+    step.prs[id] = c.k_gas * (rho_i - c.rho0);
 }
 
 __global__ __launch_bounds__(256) void k_sur_adam(float* __restrict__ w, float* __restrict__ m,
@@ -749,18 +976,28 @@ hipError_t launch_sur_pack(const float* params, __bf16* pk, float* pf, hipStream
     return hipGetLastError();
 }
 
-hipError_t launch_sur_edges(const WalkBuffers& wb, const uint32_t* batch, uint32_t nbatch,
-                            uint32_t max_items, const float4* spm, const float4* svr,
-                            const uint32_t* skey, const uint32_t* cs, const StepConsts& c,
-                            const SceneGrid* scenes, const __bf16* pk, const float* pf,
-                            const SurScales& sc, __bf16* S, __bf16* X, __bf16* Z, float* cnt,
-                            uint32_t* rowslot, hipStream_t st) {
-    const uint32_t waves = batch ? nbatch : max_items;
-    if (waves == 0) return hipSuccess;
-    uint32_t blocks = (waves + kEdgeWaves - 1) / kEdgeWaves;
-    blocks = (blocks + kXcds - 1) / kXcds * kXcds;
-    hipLaunchKernelGGL(k_sur_edges, dim3(blocks), dim3(64 * kEdgeWaves), 0, st, wb, batch,
-                       nbatch, spm, svr, skey, cs, c, scenes, pk, pf, sc, S, X, Z, cnt, rowslot);
+hipError_t launch_sur_forward(bool integrate, const WalkBuffers& wb, const uint32_t* batch,
+                              uint32_t nbatch, uint32_t max_items, const float4* spm,
+                              const float4* svr, const uint32_t* skey, const uint32_t* sid,
+                              const uint32_t* cs, uint32_t n, const StepConsts& c,
+                              const SceneGrid* scenes, const __bf16* pk, const float* pf,
+                              const SurScales& sc, const SurRows& rows, const SurStep& step,
+                              hipStream_t st) {
+    const uint32_t chunks = batch ? nbatch : max_items;
+    if (chunks == 0) return hipSuccess;
+    const uint32_t blocks = (chunks + kXcds - 1) / kXcds * kXcds;  // swizzled_block() needs it
+    if (integrate) {
+        hipLaunchKernelGGL(k_sur_forward<true>, dim3(blocks), dim3(64 * kEdgeWaves), 0, st, wb,
+                           batch, nbatch, spm, svr, skey, sid, cs, c, scenes, pk, pf, sc, rows,
+                           step);
+        // the non-finite bin holds at most n particles; the kernel reads its start on the device
+        hipLaunchKernelGGL(k_sur_nonfinite, dim3((n + 255) / 256), dim3(256), 0, st, spm, svr,
+                           sid, cs, n, c, scenes, step);
+    } else {
+        hipLaunchKernelGGL(k_sur_forward<false>, dim3(blocks), dim3(64 * kEdgeWaves), 0, st, wb,
+                           batch, nbatch, spm, svr, skey, sid, cs, c, scenes, pk, pf, sc, rows,
+                           step);
+    }
     return hipGetLastError();
 }
 
@@ -798,8 +1035,8 @@ hipError_t launch_sur_edges_bwd(const WalkBuffers& wb, const uint32_t* batch, ui
                                 const SceneGrid* scenes, const __bf16* pk, const SurScales& sc,
                                 const __bf16* DS, float* part, hipStream_t st) {
     if (nbatch == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sur_edges_bwd, dim3(nbatch), dim3(64), 0, st, wb, batch, nbatch, spm,
-                       svr, skey, cs, c, scenes, pk, sc, DS, part);
+    hipLaunchKernelGGL(k_sur_edges_bwd, dim3(nbatch), dim3(64 * kEdgeWaves), 0, st, wb, batch,
+                       nbatch, spm, svr, skey, cs, c, scenes, pk, sc, DS, part);
     return hipGetLastError();
 }
 
@@ -828,17 +1065,6 @@ hipError_t launch_sur_targets(const float4* v0, const float4* v4, uint32_t n, fl
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_sur_targets, dim3((n + 255) / 256), dim3(256), 0, st, v0, v4, n, dk,
                        inv_Tas, target);
-    return hipGetLastError();
-}
-
-hipError_t launch_sur_integrate(const float* out, const float4* spm, const float4* svr,
-                                const uint32_t* sid, const uint32_t* cs, uint32_t n,
-                                const StepConsts& c, const SceneGrid* scenes, float T, float dk,
-                                float acc_scale, float4* pm, float4* vel, float4* acc,
-                                float* rho, float* prs, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sur_integrate, dim3((n + 255) / 256), dim3(256), 0, st, out, spm, svr,
-                       sid, cs, n, c, scenes, T, dk, acc_scale, pm, vel, acc, rho, prs);
     return hipGetLastError();
 }
 

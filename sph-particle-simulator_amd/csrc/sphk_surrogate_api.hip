@@ -203,31 +203,27 @@ int prepare_density(sphk_engine* e) {
     return SPHK_OK;
 }
 
+SurRows rows_of(SurrogateState* s) {
+    SurRows r;
+    r.S = s->S;
+    r.X = s->X;
+    r.Z = s->Z;
+    r.OUT = reinterpret_cast<float4*>(s->OUT);
+    r.cnt = s->cnt;
+    r.rowslot = s->rowslot;
+    return r;
+}
+
 // Network forward for every particle of the prepared state: OUT rows by sorted slot.
 int forward_all(sphk_engine* e, SurrogateState* s) {
     const uint32_t n = (uint32_t)e->n;
     hipStream_t st = e->stream;
     if (int rc = ensure_rows(s, n, st)) return rc;
     if (int rc = pack(e, s)) return rc;
-    const SurScales sc = scales_of(e, s);
-    HIP_TRY(launch_sur_edges(e->walk, nullptr, 0, max_chunks(n, e->consts), e->spm, e->svr,
-                             e->skey, e->cell_start, e->consts, e->d_scenes, s->pk, s->pf, sc,
-                             s->S, s->X, s->Z, s->cnt, s->rowslot, st));
-    const uint32_t R = (n + 31u) / 32u * 32u;
-    SurEpiArgs ep{};
-    ep.out = s->X;
-    ep.bias = s->pf + kFB2;
-    ep.cnt = s->cnt;
-    HIP_TRY(launch_sur_gemm(kEpiM, s->S, kSurRow, s->pk + kPkW2, 128, 128, 128, R, ep, st));
-    ep = SurEpiArgs{};
-    ep.out = s->Z;
-    ep.bias = s->pf + kFBn1;
-    HIP_TRY(launch_sur_gemm(kEpiZ, s->X, kSurRow, s->pk + kPkWn1, kSurRow, kSurRow, 128, R, ep,
-                            st));
-    ep = SurEpiArgs{};
-    ep.out = s->OUT;
-    ep.bias = s->pf + kFBn2;
-    HIP_TRY(launch_sur_gemm(kEpiOut, s->Z, kSurRow, s->pk + kPkWn2, 128, 128, 32, R, ep, st));
+    HIP_TRY(launch_sur_forward(false, e->walk, nullptr, 0, max_chunks(n, e->consts), e->spm,
+                               e->svr, e->skey, e->sid, e->cell_start, n, e->consts,
+                               e->d_scenes, s->pk, s->pf, scales_of(e, s), rows_of(s),
+                               SurStep{}, st));
     return SPHK_OK;
 }
 
@@ -266,23 +262,9 @@ int loss_grad_batch(sphk_engine* e, SurrogateState* s, uint32_t nb, float* loss)
         *loss = 0.0f;
         return SPHK_OK;
     }
-    HIP_TRY(launch_sur_edges(e->walk, s->d_batch, nb, 0, e->spm, e->svr, e->skey, e->cell_start,
-                             e->consts, e->d_scenes, s->pk, s->pf, sc, s->S, s->X, s->Z, s->cnt,
-                             s->rowslot, st));
-    SurEpiArgs ep{};
-    ep.out = s->X;
-    ep.bias = s->pf + kFB2;
-    ep.cnt = s->cnt;
-    HIP_TRY(launch_sur_gemm(kEpiM, s->S, kSurRow, s->pk + kPkW2, 128, 128, 128, R, ep, st));
-    ep = SurEpiArgs{};
-    ep.out = s->Z;
-    ep.bias = s->pf + kFBn1;
-    HIP_TRY(launch_sur_gemm(kEpiZ, s->X, kSurRow, s->pk + kPkWn1, kSurRow, kSurRow, 128, R, ep,
-                            st));
-    ep = SurEpiArgs{};
-    ep.out = s->OUT;
-    ep.bias = s->pf + kFBn2;
-    HIP_TRY(launch_sur_gemm(kEpiOut, s->Z, kSurRow, s->pk + kPkWn2, 128, 128, 32, R, ep, st));
+    HIP_TRY(launch_sur_forward(false, e->walk, s->d_batch, nb, 0, e->spm, e->svr, e->skey,
+                               e->sid, e->cell_start, (uint32_t)e->n, e->consts, e->d_scenes,
+                               s->pk, s->pf, sc, rows_of(s), SurStep{}, st));
     // loss and its row gradients
     const uint32_t blocks = (R + 255u) / 256u;
     if (int rc = ensure_red(s, 2ull * blocks)) return rc;
@@ -305,7 +287,7 @@ int loss_grad_batch(sphk_engine* e, SurrogateState* s, uint32_t nb, float* loss)
     *loss = (float)(sq / (3.0 * rows));
     const float gscale = (float)(1.0 / (3.0 * rows));
     // backward through the node MLP and the hoisted message layer
-    ep = SurEpiArgs{};
+    SurEpiArgs ep{};
     ep.out = s->DZ;
     ep.z = s->Z;
     HIP_TRY(launch_sur_gemm(kEpiDZ, s->DOUT, kSurOutRow, s->pk + kPkWn2T, 32, 32, 128, R, ep, st));
@@ -318,7 +300,8 @@ int loss_grad_batch(sphk_engine* e, SurrogateState* s, uint32_t nb, float* loss)
     HIP_TRY(launch_sur_gemm(kEpiDS, s->DM, 128, s->pk + kPkW2T, 128, 128, 128, R, ep, st));
     // weight gradients (partials per slab of rows, then a fixed-order reduction)
     const uint32_t slabs = (R + kRowsPerSlab - 1) / kRowsPerSlab;
-    const uint64_t need = std::max<uint64_t>((uint64_t)slabs * 128 * kSurRow, (uint64_t)nb * 128 * 16);
+    const uint64_t need =
+        std::max<uint64_t>((uint64_t)slabs * 128 * kSurRow, 2ull * nb * 128 * 16);
     if (int rc = grow_buf(&s->part, s->part_cap, need)) return rc;
     HIP_TRY(launch_sur_wgrad(s->DOUT, kSurOutRow, 3, s->Z, kSurRow, 129, R, kRowsPerSlab, slabs,
                              s->part, st));
@@ -334,8 +317,8 @@ int loss_grad_batch(sphk_engine* e, SurrogateState* s, uint32_t nb, float* loss)
                               s->grad, st));
     HIP_TRY(launch_sur_edges_bwd(e->walk, s->d_batch, nb, e->spm, e->svr, e->skey, e->cell_start,
                                  e->consts, e->d_scenes, s->pk, sc, s->DS, s->part, st));
-    HIP_TRY(launch_sur_reduce(s->part, nb, 128, 16, 128, 8, 8, kOffW1, kOffB1, gscale, s->grad,
-                              st));
+    HIP_TRY(launch_sur_reduce(s->part, 2u * nb, 128, 16, 128, kSurNF, kSurNF, kOffW1, kOffB1,
+                              gscale, s->grad, st));
     return SPHK_OK;
 }
 
@@ -401,15 +384,24 @@ int make_targets(sphk_engine* e, SurrogateState* s, float dt) {
     return SPHK_OK;
 }
 
-// One network macro-step of the prepared-or-not current state.
+// One network macro-step of the current state (forward and integration in one pass).
 int macro_step(sphk_engine* e, SurrogateState* s, float dt) {
     if (int rc = prepare_density(e)) return rc;
-    if (int rc = forward_all(e, s)) return rc;
-    const float T = dt * (float)s->cfg.substeps;
-    HIP_TRY(launch_sur_integrate(s->OUT, e->spm, e->svr, e->sid, e->cell_start, (uint32_t)e->n,
-                                 e->consts, e->d_scenes, T, damping_power(e, s),
-                                 s->cfg.acc_scale, e->pm, e->vel, e->acc, e->rho, e->prs,
-                                 e->stream));
+    if (int rc = pack(e, s)) return rc;
+    const uint32_t n = (uint32_t)e->n;
+    SurStep step;
+    step.T = dt * (float)s->cfg.substeps;
+    step.dk = damping_power(e, s);
+    step.pm = e->pm;
+    step.vel = e->vel;
+    step.acc = e->acc;
+    step.rho = e->rho;
+    step.prs = e->prs;
+    SurRows none{};
+    HIP_TRY(launch_sur_forward(true, e->walk, nullptr, 0, max_chunks(n, e->consts), e->spm,
+                               e->svr, e->skey, e->sid, e->cell_start, n, e->consts,
+                               e->d_scenes, s->pk, s->pf, scales_of(e, s), none, step,
+                               e->stream));
     HIP_TRY(launch_advance_time(e->fscal + 1, nullptr, dt, e->stream, s->cfg.substeps));
     e->step_count += (uint64_t)s->cfg.substeps;
     after_positions_moved(e);
@@ -494,7 +486,8 @@ int sphk_surrogate_init(sphk_engine* e, const sphk_surrogate_config* cfg, uint64
     std::vector<float> p(kSurParams, 0.0f);
     struct Layer {
         uint32_t off, fan_out, fan_in;
-    } layers[] = {{kOffW1, 128, 8}, {kOffW2, 128, 128}, {kOffWn1, 128, kSurNodeIn}, {kOffWn2, 3, 128}};
+    } layers[] = {{kOffW1, 128, kSurNF}, {kOffW2, 128, 128}, {kOffWn1, 128, kSurNodeIn},
+                  {kOffWn2, 3, 128}};
     for (const Layer& L : layers) {
         const double a = std::sqrt(6.0 / (double)(L.fan_in + L.fan_out));
         for (uint32_t k = 0; k < L.fan_out * L.fan_in; ++k)

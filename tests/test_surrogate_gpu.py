@@ -7,9 +7,9 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 F32 = np.float32
-BLOCKS = [("W1", 0, 1024), ("b1", 1024, 1152), ("W2", 1152, 17536), ("b2", 17536, 17664),
-          ("Wn1", 17664, 34560), ("bn1", 34560, 34688), ("Wn2", 34688, 35072),
-          ("bn2", 35072, 35075)]
+BLOCKS = [("W1", 0, 1152), ("b1", 1152, 1280), ("W2", 1280, 17664), ("b2", 17664, 17792),
+          ("Wn1", 17792, 34688), ("bn1", 34688, 34816), ("Wn2", 34816, 35200),
+          ("bn2", 35200, 35203)]
 
 
 def _engine(sphk, scene, formulation=1):
@@ -45,7 +45,7 @@ def _active_params(e, seed=0):
 
 
 def test_param_count_and_xavier_init(sphk, scenes):
-    assert sphk.surrogate_param_count() == 35075
+    assert sphk.surrogate_param_count() == 35203
     e = _engine(sphk, scenes.small_block(4, 4, 4))
     e.surrogate_init(seed=11)
     p = e.surrogate_params()
@@ -54,7 +54,7 @@ def test_param_count_and_xavier_init(sphk, scenes):
         if name.startswith("b"):
             assert np.all(blk == 0)
         else:
-            fan_out, fan_in = {"W1": (128, 8), "W2": (128, 128), "Wn1": (128, 132),
+            fan_out, fan_in = {"W1": (128, 9), "W2": (128, 128), "Wn1": (128, 132),
                                "Wn2": (3, 128)}[name]
             lim = np.sqrt(6.0 / (fan_in + fan_out))
             assert np.abs(blk).max() <= lim and np.abs(blk).max() > 0.9 * lim

@@ -35,8 +35,8 @@ def test_bf16_round_matches_c_and_ties(oracle):
 
 
 def test_param_count(oracle):
-    # W1 128x8 + b1 + W2 128x128 + b2 + Wn1 128x132 + bn1 + Wn2 3x128 + bn2
-    assert oracle.sur_nparams(oracle.sur_cfg()) == 35075
+    # W1 128x9 + b1 + W2 128x128 + b2 + Wn1 128x132 + bn1 + Wn2 3x128 + bn2
+    assert oracle.sur_nparams(oracle.sur_cfg()) == 35203
 
 
 def _per_edge_reference(oracle, cfg, params, pos, vel, rho, offs, idx):
@@ -52,7 +52,7 @@ def _per_edge_reference(oracle, cfg, params, pos, vel, rho, offs, idx):
         k += size
         return out
 
-    W1, b1 = take((h1, 8)), take((h1,))
+    W1, b1 = take((h1, 9)), take((h1,))
     W2, b2 = take((h2, h1)), take((h2,))
     Wn1, bn1 = take((h3, h2 + 4)), take((h3,))
     Wn2, bn2 = take((3, h3)), take((3,))
@@ -76,7 +76,8 @@ def _per_edge_reference(oracle, cfg, params, pos, vel, rho, offs, idx):
             else:
                 w = f32(0)
             e = np.concatenate([r * f32(cfg.inv_r), (vel[i] - vel[j]) * f32(cfg.inv_v),
-                                [rl * f32(cfg.inv_r), f32(w) * f32(cfg.w_norm)]]).astype(np.float64)
+                                [rl * f32(cfg.inv_r), f32(w) * f32(cfg.w_norm),
+                                 (rho[j] - f32(cfg.rho0)) * f32(cfg.rho_scale)]]).astype(np.float64)
             m += cfg.msg_scale * (W2 @ np.maximum(W1 @ e + b1, 0.0) + b2)
         x = np.concatenate([m, [(rho[i] - f32(cfg.rho0)) * f32(cfg.rho_scale)],
                             vel[i] * f32(cfg.inv_v)])
@@ -110,9 +111,9 @@ def test_forward_closed_form_without_edge_weights(oracle):
     cfg = oracle.sur_cfg(h1=4, h2=3, h3=2, bf16=False)
     pos, vel, rho, offs, idx = _scene(oracle, n=30, seed=4)
     params = _params(oracle, cfg, seed=8)
-    params[:4 * 8] = 0.0
+    params[:4 * 9] = 0.0
     _, S, cnt = oracle.sur_forward(cfg, params, pos, vel, rho, offs, idx, want_S=True)
-    b1 = params[32:36].astype(np.float64)
+    b1 = params[36:40].astype(np.float64)
     n_edges = np.diff(offs) - 1  # self excluded
     np.testing.assert_array_equal(cnt, n_edges)
     np.testing.assert_allclose(S, cfg.msg_scale * n_edges[:, None] * np.maximum(b1, 0)[None, :],
