@@ -31,6 +31,7 @@ METRIC = "particle-updates/s @1M dam break: plain GPU SPH vs surrogate at ≤1% 
 ALG_BYTES = {"density": 16.0, "force": 52.0, "step": 149.0}
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
 
 
 def parse():
@@ -161,8 +162,12 @@ def surrogate_line(sphk, sph_scenes, snapshots, macro_steps):
     e.synchronize()
     t_train = time.perf_counter() - t0
     e.snapshot_save()
-    # warm both paths (first-use allocations), then restore
+    # warm both paths (first-use allocations), then restore; the plain step also counts the
+    # neighbour lists of the trained-to state (edges of the network's layer 1)
+    e.reset_stats()
     e.step(dt)
+    st = e.stats()
+    edges = float(st.total_neighbors - st.total_neighbor_queries)  # self excluded
     e.surrogate_macro_step(dt)
     e.snapshot_restore()
     e.synchronize()
@@ -221,7 +226,15 @@ def surrogate_line(sphk, sph_scenes, snapshots, macro_steps):
         "training": {"snapshots": snapshots, "batch_particles": 1024 * 64, "seconds": t_train,
                      "loss_first": float(curve[0]), "loss_last": float(curve[-20:].mean())},
         "dtype": "bf16 MFMA operands, f32 accumulation",
+        # useful MFMA work of one network evaluation: layer 1 over every edge (K = 10 of the
+        # 16-wide operand: 9 features + bias) and the node MLP per particle
+        "mfma": {"useful_flop_per_macro_step": 2.0 * edges * 10 * 128 +
+                 2.0 * n * (128 * 128 + 132 * 128 + 3 * 128),
+                 "achieved_tflops_over_macro_step": (2.0 * edges * 10 * 128 + 2.0 * n * (
+                     128 * 128 + 132 * 128 + 3 * 128)) / t_macro / 1e12,
+                 "peak_tflops": BF16_PEAK_TFLOPS},
     }
+    out["mfma"]["frac"] = out["mfma"]["achieved_tflops_over_macro_step"] / BF16_PEAK_TFLOPS
     del e
     return out
 
