@@ -358,6 +358,177 @@ __device__ __forceinline__ void store_words(uint32_t item, int seg_base, int nw,
 }
 
 
+// ---------------------------------------------------------------------------
+// Pair walk: two lanes per target (l and l ^ 32), 32 targets per wavefront.  The non-empty
+// words of each target are compacted into an LDS list; the lower lane takes neighbour bits
+// from the front, the upper one from the back, each its share of the count, so every lane
+// walks continuously (no per-segment maxima) and the two never meet.
+// ---------------------------------------------------------------------------
+// Wave max of a per-lane count.
+__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(x, off);
+        x = o > x ? o : x;
+    }
+    return uni(x);
+}
+
+constexpr int kCols = 32;       // target columns of a wavefront (two lanes per target)
+constexpr int kListCap = 72;    // non-empty words per target column held at once
+constexpr int kBatch = 16;      // stored words requested per batch while building a list
+
+// LDS of one wavefront's walk: per target column, its non-empty words (mask, first slot)
+// in candidate order, entry i of column c at list[i * 32 + c].
+struct alignas(16) WalkLds {
+    uint2 list[kListCap * kCols];
+};
+
+// Appends the words of a window to the column lists, up to kListCap non-empty words per
+// column.  Stored words are read from the density pass; otherwise candidates are tested here
+// (phase 1 with the contracted filter).  Returns false when the chunk has no words left.
+struct ListBuilder {
+    int wpos = 0;        // next stored word
+    uint32_t n_ne = 0;   // non-empty words of this lane's column in the window
+    uint32_t cnt = 0;    // neighbour bits of this lane's column in the window
+};
+
+__device__ __forceinline__ bool build_list(ListBuilder& lb, bool stored, int total_words,
+                                           const WalkBuffers& wb, uint32_t item,
+                                           uint32_t clane, Cursor& cu, const Chunk& ch,
+                                           const float4* spm, const uint32_t* cs_,
+                                           const StepConsts& c, float4 pi, uint32_t lane,
+                                           WalkLds& L) {
+    const uint32_t col = lane & 31u;
+    const bool writer = lane < 32u;
+    lb.n_ne = 0u;
+    lb.cnt = 0u;
+    bool any = false;
+    if (stored) {
+        const size_t base = (size_t)item * kMaxWords;
+        // a window of up to kListCap words cannot overflow a column list
+        const int end = min(total_words, lb.wpos + kListCap);
+        while (lb.wpos < end) {
+            const int nb = min(kBatch, end - lb.wpos);
+            uint32_t wv[kBatch];
+            uint32_t jv[kBatch];
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                const int w = lb.wpos + min(k, nb - 1);
+                wv[k] = wb.mask[(base + w) * 64 + clane];
+                jv[k] = uni(wb.wj0[base + w]);
+            }
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                if (k < nb && wv[k] != 0u) {
+                    if (writer) L.list[lb.n_ne * kCols + col] = make_uint2(wv[k], jv[k]);
+                    ++lb.n_ne;
+                    lb.cnt += (uint32_t)__builtin_popcount(wv[k]);
+                }
+            }
+            lb.wpos += nb;
+            any = true;
+        }
+    } else {
+        while (__ballot(lb.n_ne >= (uint32_t)kListCap) == 0ull && cursor_settle(cu, ch, cs_, c)) {
+            const uint32_t j0 = uni(cu.j);
+            const uint32_t cnt = uni(min(32u, cu.end - j0));
+            uint32_t word = 0;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4 q = spm[j0 + k];
+                word = push_bit(word, dist2_filter(pi.x, pi.y, pi.z, q.x, q.y, q.z), c.r2_filter);
+            }
+            word <<= (32u - cnt) & 31u;  // left-aligned: candidate k in bit 31 - k
+            if (word != 0u) {
+                if (writer) L.list[lb.n_ne * kCols + col] = make_uint2(word, j0);
+                ++lb.n_ne;
+                lb.cnt += (uint32_t)__builtin_popcount(word);
+            }
+            cu.j = j0 + cnt;
+            any = true;
+        }
+    }
+    return any;
+}
+
+// Walk of one lane over its target's list: the lower lane of a target (p = 0) takes bits
+// from the front, the upper one (p = 1) from the back, each its share of the count, so the
+// two never meet.  `e` is the list entry at `idx`, read one step ahead.
+struct PairWalk {
+    uint32_t m = 0u, jb = 0u;
+    int idx = 0, dir = 1;
+    uint2 e;
+    __device__ __forceinline__ void begin(uint32_t p, uint32_t n_ne) {
+        m = 0u;
+        jb = 0u;
+        dir = p ? -1 : 1;
+        idx = p ? (int)n_ne - 1 : 0;
+    }
+    __device__ __forceinline__ void request(const WalkLds& L, uint32_t col) {
+        e = L.list[(uint32_t)max(idx, 0) * kCols + col];
+    }
+    __device__ __forceinline__ uint32_t next(uint32_t p) {
+        const bool refill = m == 0u;
+        m = refill ? e.x : m;
+        jb = refill ? e.y : jb;
+        idx += refill ? dir : 0;
+        const uint32_t bit = p ? (uint32_t)__builtin_ctz(m) : 31u - (uint32_t)__builtin_clz(m);
+        m ^= 1u << bit;
+        return jb + (31u - bit);
+    }
+};
+
+// Walks every neighbour j != i of the wavefront's 32 targets, two lanes per target, in
+// uniform steps; body(edge, pj, vj, d2) is called once per step (edge = false on lanes with
+// nothing left).  Neighbour data is requested one step ahead (vj only with kVel).
+template <bool kVel = true, class Body>
+__device__ __forceinline__ void walk_pairs(uint32_t item, const Chunk& ch, uint32_t s,
+                                           bool active, float4 pi, uint32_t clane,
+                                           const WalkBuffers& wb, const float4* spm,
+                                           const float4* svr, const uint32_t* cs_,
+                                           const StepConsts& c, WalkLds& L, uint32_t lane,
+                                           Body&& body) {
+    const uint32_t col = lane & 31u, p = lane >> 5;
+    const int total_words = (int)uni(wb.nwords[item]);
+    const bool stored = total_words <= kMaxWords;
+    const uint32_t s_safe = active ? s : ch.s0;
+    Cursor cu;
+    if (!stored) cursor_begin(cu, ch, cs_, c);
+    ListBuilder lb;
+    for (;;) {
+        if (!build_list(lb, stored, total_words, wb, item, clane, cu, ch, spm, cs_, c, pi, lane,
+                        L))
+            break;
+        wave_sync();
+        const uint32_t quota = p ? lb.cnt >> 1 : (lb.cnt + 1u) >> 1;
+        const uint32_t tmax = wave_max(quota);
+        PairWalk pw;
+        pw.begin(p, lb.n_ne);
+        pw.request(L, col);
+        uint32_t jn = s_safe;
+        if (quota > 0u) jn = pw.next(p);
+        pw.request(L, col);
+        float4 pn = spm[jn];
+        float4 vn = kVel ? svr[jn] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint32_t t = 0; t < tmax; ++t) {
+            const uint32_t j = jn;
+            const bool has = t < quota;
+            const float4 pj = pn;
+            const float4 vj = vn;
+            jn = s_safe;
+            if (t + 1u < quota) jn = pw.next(p);
+            pw.request(L, col);
+            pn = spm[jn];
+            if (kVel) vn = svr[jn];
+            const float d2 = dist2_exact(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
+            const bool edge = has && j != s && d2 <= c.r2;  // the reference's test, self excluded
+            body(edge, pj, vj, d2);
+        }
+        wave_sync();
+    }
+}
+
+
 template <int kSeg, int kWaves>
 struct WalkShape {
     static constexpr int seg = kSeg;
