@@ -29,6 +29,10 @@ METRIC = "particle-updates/s @1M dam break: plain GPU SPH vs surrogate at ≤1% 
 #   force pass:    read x,y,z,v,rho 28 + write x,v 24                = 52
 #   whole step:    keys 16 + histogram 4 + scan ~0.7 + reorder 60 + 16 + 52 ~= 149
 ALG_BYTES = {"density": 16.0, "force": 52.0, "step": 149.0}
+# HBM bytes the force walk moves per particle on the 1M dam break, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE scaled by a 256 MiB device copy (fetch x2.0, write x1.0); see
+# profiles/force_walk_dam_break_1M_traffic.md.  Mostly the stored neighbour words it reads back.
+FORCE_TRAFFIC_BYTES_PER_PARTICLE = 499.0
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
@@ -50,6 +54,12 @@ def parse():
                     help="particle-count scale of the 64-scene ensemble (1.0 = full size)")
     ap.add_argument("--continuous", type=int, default=0,
                     help="also time N continuous steps of the consistent formulation at dt=1e-4")
+    ap.add_argument("--walk-shapes", type=str, default=None,
+                    help="density,force launch shapes (indices of sphk.walk_shape_names()); "
+                         "skips the autotuner")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device three times "
+                         "(reference launches for HBM byte counters)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned macro-step surrogate line")
     ap.add_argument("--surrogate-snapshots", type=int, default=2000,
@@ -261,7 +271,10 @@ def main():
     e = sphk.Engine(scene.n, scene.params(sphk))
     e.upload(scene.pos, scene.vel)
     tune = None
-    if not args.no_autotune:
+    if args.walk_shapes:
+        d, f = (int(x) for x in args.walk_shapes.split(","))
+        e.set_walk_shapes(d, f)
+    elif not args.no_autotune:
         import sph_autotune
 
         tune = sph_autotune.autotune(e, scene.dt, steps=3, use_cache=False)
@@ -328,7 +341,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": FORCE_TRAFFIC_BYTES_PER_PARTICLE * n if args.config == 1 else None,
             "alg_bytes_per_update": ALG_BYTES["force"],
             "kernel_ms": t_force * 1e3,
         },
@@ -366,6 +379,14 @@ def main():
     if not args.no_surrogate and args.config == 1:
         out["surrogate"] = surrogate_line(sphk, sph_scenes, args.surrogate_snapshots,
                                           args.surrogate_macro)
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(args.calibrate_copy * 262144, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     if not args.no_cpu_baseline:
         del e
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_repeats)

@@ -191,10 +191,24 @@ __global__ void k_bbox(const float4* __restrict__ pm, uint32_t begin, uint32_t e
         unsigned b = __float_as_uint(f);
         return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
     };
-    if (found) {
+    // wavefront reduction first: one set of atomics per wavefront, not per thread
+    unsigned olo[3], ohi[3];
+    for (int d = 0; d < 3; ++d) {
+        olo[d] = ord(lo[d]);
+        ohi[d] = ord(hi[d]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
         for (int d = 0; d < 3; ++d) {
-            atomicMin(&out7[d], ord(lo[d]));
-            atomicMax(&out7[3 + d], ord(hi[d]));
+            const unsigned a = __shfl_xor(olo[d], off), b = __shfl_xor(ohi[d], off);
+            olo[d] = a < olo[d] ? a : olo[d];
+            ohi[d] = b > ohi[d] ? b : ohi[d];
+        }
+    }
+    const bool any = __ballot(found) != 0ull;
+    if (any && (threadIdx.x & 63u) == 0u) {
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&out7[d], olo[d]);
+            atomicMax(&out7[3 + d], ohi[d]);
         }
         atomicOr(&out7[6], 1u);
     }
