@@ -10,6 +10,8 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include "export.h"
+#include "kernels.h"
 #include "sph_engine.h"
 
 namespace py = pybind11;
@@ -192,6 +194,31 @@ PYBIND11_MODULE(sph, m) {
             return sph::create_boundary_box(c, s, spacing, mass);
         },
         py::arg("center"), py::arg("size"), py::arg("spacing"), py::arg("mass") = 1.0f);
+
+    // Additions of this build (no binding in the reference module): the kernel classes of
+    // src/kernels.h and the CSV time series of examples/dam_break.cpp.
+    py::enum_<sph::KernelType>(m, "KernelType")
+        .value("CUBIC_SPLINE", sph::KernelType::CUBIC_SPLINE)
+        .value("WENDLAND_C2", sph::KernelType::WENDLAND_C2)
+        .value("GAUSSIAN", sph::KernelType::GAUSSIAN);
+    py::class_<sph::Kernel>(m, "Kernel")
+        .def("W", &sph::Kernel::W)
+        .def("gradW", &sph::Kernel::gradW)
+        .def("laplacianW", &sph::Kernel::laplacianW)
+        .def("get_smoothing_length", &sph::Kernel::get_smoothing_length)
+        .def("set_smoothing_length", &sph::Kernel::set_smoothing_length)
+        .def("get_support_radius", &sph::Kernel::get_support_radius);
+    m.def("create_kernel", &sph::create_kernel, py::arg("type"), py::arg("smoothing_length"));
+    m.def(
+        "cubic_W_2D",
+        [](float h, float x, float y) { return sph::CubicSplineKernel(h).W_2D(sph::vec2(x, y)); });
+    m.def("cubic_W_1D", [](float h, float r) { return sph::CubicSplineKernel(h).W_1D(r); });
+    m.def("estimate_smoothing_length", &sph::kernel_utils::estimate_smoothing_length,
+          py::arg("particle_spacing"), py::arg("neighbor_count") = 50.0f);
+    py::class_<sph::utils::CsvTimeSeries>(m, "CsvTimeSeries")
+        .def(py::init<const std::string&>())
+        .def("is_open", &sph::utils::CsvTimeSeries::is_open)
+        .def("record", &sph::utils::CsvTimeSeries::record);
 
     m.attr("__version__") = "1.0.0";
 }

@@ -31,7 +31,7 @@ HIP_FLAGS = [
 
 SPHK_SOURCES = ["csrc/sphk_kernels.hip", "csrc/sphk_walk.hip", "csrc/sphk_api.hip",
                 "csrc/sphk_surrogate.hip", "csrc/sphk_surrogate_api.hip"]
-ENGINE_SOURCES = ["host/particle.cpp", "host/sph_engine.cpp"]
+ENGINE_SOURCES = ["host/particle.cpp", "host/sph_engine.cpp", "host/kernels.cpp", "host/export.cpp"]
 
 
 def _run(cmd, cwd=HERE):
