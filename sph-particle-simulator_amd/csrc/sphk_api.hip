@@ -325,17 +325,19 @@ int ensure_grid(sphk_engine* e) {
     e->consts.G = (uint32_t)G;
     e->consts.ncols = (uint32_t)ncols;
     e->consts.nscenes = (uint32_t)S;
+    e->consts.pad_slot = (uint32_t)e->cap;
     // chunk work lists and neighbour words: at most ceil(C/64) + one partial chunk per column
     const uint64_t chunks = (e->cap + 63) / 64 + ncols;
     WalkBuffers& wb = e->walk;
     if (chunks > wb.chunk_cap) {
-        for (uint32_t* p : {wb.items, wb.mask, wb.wj0, wb.nwords})
+        for (uint32_t* p : {wb.items, wb.mask, wb.wj0, wb.nwords, wb.groups})
             if (p) HIP_TRY(hipFree(p));
         size_t want = std::max<size_t>(chunks, wb.chunk_cap * 5 / 4);
         HIP_TRY(dalloc(&wb.items, want));
         HIP_TRY(dalloc(&wb.mask, want * kMaxWords * 64));
         HIP_TRY(dalloc(&wb.wj0, want * kMaxWords));
         HIP_TRY(dalloc(&wb.nwords, want));
+        HIP_TRY(dalloc(&wb.groups, want));
         wb.chunk_cap = want;
         ++e->buffer_gen;
     }
@@ -478,10 +480,11 @@ int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev) {
     if (adaptive) HIP_TRY(launch_cfl(e->vel, e->acc, n, e->params, e->uscal + 1, e->fscal, st));
     if (int rc = build_sorted(e)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
-    HIP_TRY(launch_density_walk(e->walk, e->spm, e->svr, e->skey, e->cell_start, n, e->consts,
-                                e->d_scenes, e->nstats, e->shape_density, st));
+    HIP_TRY(launch_density_walk(e->walk, e->spm, e->svr, e->sfa, e->sfc, e->skey, e->cell_start,
+                                n, e->consts, e->d_scenes, e->nstats, e->shape_density, st));
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
-    HIP_TRY(launch_force_walk(e->walk, e->spm, e->svr, e->skey, e->sid, e->cell_start, n,
+    HIP_TRY(launch_force_walk(e->walk, e->spm, e->svr, e->sfa, e->sfc, e->skey, e->sid,
+                              e->cell_start, n,
                               e->consts, e->d_scenes, adaptive ? e->fscal : nullptr, dt, e->pm,
                               e->vel, e->acc, e->rho, e->prs, e->shape_force, st));
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
@@ -564,8 +567,19 @@ int sphk_create(size_t capacity, sphk_engine** out) {
     HIP_TRY(dalloc(&e->vals, C));
     HIP_TRY(dalloc(&e->vals_alt, C));
     HIP_TRY(dalloc(&e->hist, ((size_t)1 << kRadixMaxBits) * ((C + kRadixTile - 1) / kRadixTile)));
-    HIP_TRY(dalloc(&e->spm, C));
-    HIP_TRY(dalloc(&e->svr, C));
+    HIP_TRY(dalloc(&e->spm, C + 1));
+    HIP_TRY(dalloc(&e->svr, C + 1));
+    HIP_TRY(dalloc(&e->sfa, C + 1));
+    HIP_TRY(dalloc(&e->sfc, C + 1));
+    {
+        // padding slot: zero mass and weights, 1e15 away from any particle
+        const float4 far_row = make_float4(1e15f, 1e15f, 1e15f, 0.0f);
+        const float4 zero_row = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        HIP_TRY(hipMemcpy(e->spm + C, &far_row, sizeof(float4), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->sfa + C, &far_row, sizeof(float4), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->svr + C, &zero_row, sizeof(float4), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->sfc + C, &zero_row, sizeof(float4), hipMemcpyHostToDevice));
+    }
     HIP_TRY(dalloc(&e->fscal, 4));
     HIP_TRY(dalloc(&e->uscal, 4));
     HIP_TRY(dalloc(&e->nstats, 2));
@@ -588,12 +602,12 @@ int sphk_destroy(sphk_engine* e) {
     void* ptrs[] = {e->pm,   e->vel,      e->acc,  e->rho,      e->prs,  e->snap_pm,
                     e->snap_vel, e->snap_acc, e->keys, e->keys_alt, e->vals, e->vals_alt,
                     e->hist, e->spm,      e->svr,  e->cell_start, e->fscal, e->uscal,
-                    e->nstats};
+                    e->nstats, e->sfa, e->sfc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : {(void*)e->walk.items, (void*)e->walk.nitems, (void*)e->walk.colcnt,
                     (void*)e->walk.mask, (void*)e->walk.wj0, (void*)e->walk.nwords,
-                    (void*)e->scan_scratch, (void*)e->d_scenes})
+                    (void*)e->walk.groups, (void*)e->scan_scratch, (void*)e->d_scenes})
         if (p) (void)hipFree(p);
     if (e->sur) surrogate_free(e->sur);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);

@@ -94,8 +94,13 @@ struct sphk_engine {
     WalkBuffers walk;
     int shape_density = 0;  // walk launch shapes (walk_shape_name), set by sphk_set_walk_shapes
     int shape_force = 0;
+    // sorted copies, capacity + 1 rows (the last is the padding slot StepConsts::pad_slot):
+    // spm (x, y, z, m), svr (vx, vy, vz, rho), and the force walk's neighbour rows written by
+    // the density walk: sfa (x, y, z, m/rho), sfc (vx, vy, vz, m P / (2 rho^2))
     float4* spm = nullptr;
     float4* svr = nullptr;
+    float4* sfa = nullptr;
+    float4* sfc = nullptr;
     uint32_t* cell_start = nullptr;
     uint64_t cell_start_cap = 0;
     uint32_t* skey = nullptr;  // -> keys or keys_alt after the sort

@@ -36,6 +36,8 @@ struct StepConsts {
                       // non-finite coordinate
     uint32_t ncols;   // (x, y) columns of all scene grids
     uint32_t nscenes;
+    uint32_t pad_slot;  // sorted slot past every particle (index = capacity): zero mass and
+                        // weights, far away; the walks read it in place of a neighbour
 };
 
 // Dense cell grid of one scene (z fastest) and its place in the global cell/column
@@ -75,6 +77,7 @@ struct WalkBuffers {
     uint32_t* mask = nullptr;     // [chunks][kMaxWords][64] neighbour bits
     uint32_t* wj0 = nullptr;      // [chunks][kMaxWords] first slot of each word
     uint32_t* nwords = nullptr;   // [chunks] words of each chunk
+    uint32_t* groups = nullptr;   // [chunks] word index of column groups 1-3, 8 bits each
     size_t chunk_cap = 0;
     size_t col_cap = 0;
 };
@@ -128,11 +131,13 @@ hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, const 
                          uint32_t* cnt_scratch, uint32_t* scan_scratch, uint32_t* items,
                          uint32_t* nitems, hipStream_t st);
 hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, float4* svr,
-                               const uint32_t* skey, const uint32_t* cell_start, uint32_t n,
+                               float4* sfa, float4* sfc, const uint32_t* skey,
+                               const uint32_t* cell_start, uint32_t n,
                                const StepConsts& c, const SceneGrid* scenes,
                                unsigned long long* nbr_stats, int shape, hipStream_t st);
 hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const float4* svr,
-                             const uint32_t* skey, const uint32_t* sid,
+                             const float4* sfa, const float4* sfc, const uint32_t* skey,
+                             const uint32_t* sid,
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
                              const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
                              float4* vel, float4* acc, float* rho, float* prs, int shape,

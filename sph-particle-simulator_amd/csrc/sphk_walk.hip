@@ -80,15 +80,35 @@ uint32_t max_chunks(uint32_t n, const StepConsts& c) { return (n + 63u) / 64u + 
 
 // ---------------------------------------------------------------------------
 // density: rho_i = m_i W(0) + sum_{j != i} m_j W(r_ij)   (compute_density,
-// src/sph_engine.cpp:370-383).  Stores the neighbour words of every chunk.
+// src/sph_engine.cpp:370-383).  The self pair is one of the words' bits (d = 0), so the sum
+// runs over the set bits, self included.  Stores the neighbour words of every chunk and the
+// force walk's neighbour rows of every particle.
 // ---------------------------------------------------------------------------
+template <bool kCount, int kSeg>
+__device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
+                                              const uint32_t* __restrict__ lmask,
+                                              const uint32_t* __restrict__ lwj0, uint32_t lane,
+                                              const float4* __restrict__ spm, const float4& pi,
+                                              const StepConsts& c, float acc, uint32_t& cnt) {
+    for (uint32_t t = 0; t < steps; ++t) {
+        const uint32_t j = bw.next(lmask, lwj0, lane);
+        const float4 q = spm[j];
+        const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
+        if (kCount) cnt += d2 <= c.r2 ? 1u : 0u;  // the reference's test
+        // pairs the filter let through beyond the radius have q >= 2 and weigh zero
+        acc += q.w * w_shape(__builtin_amdgcn_sqrtf(d2) * c.inv_h);
+    }
+    return acc;
+}
+
 template <class Shape>
 __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
-    const float4* __restrict__ spm, float4* __restrict__ svr, const uint32_t* __restrict__ skey,
+    const float4* __restrict__ spm, float4* __restrict__ svr, float4* __restrict__ sfa,
+    float4* __restrict__ sfc, const uint32_t* __restrict__ skey,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     uint32_t* __restrict__ gmask, uint32_t* __restrict__ gwj0, uint32_t* __restrict__ gnw,
-    unsigned long long* __restrict__ stats) {
+    uint32_t* __restrict__ ggrp, unsigned long long* __restrict__ stats) {
     constexpr int kSeg = Shape::seg;
     __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
     __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
@@ -102,8 +122,10 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     bool active;
     float4 pi;
     if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
+    init_sentinel<kSeg>(lmask, lwj0, lane, c);
     Cursor cu;
     cursor_begin(cu, ch, cs, c);
+    if (lane == 0) ggrp[item] = pack_groups(cu);
     float acc = 0.0f;
     uint32_t cnt = 0;
     int seg_base = 0;
@@ -116,21 +138,28 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
         store_words(item, seg_base, nw, lmask, lwj0, gmask, gwj0, lane);
         BitWalk<kSeg> bw;
         bw.ne = sl.nonempty;
-        for (uint32_t t = 0; t < sl.count; ++t) {
-            const uint32_t j = bw.next(lmask, lwj0, lane);
-            const float4 q = spm[j];
-            const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
-            const bool nb = d2 <= c.r2;  // the reference's test
-            cnt += nb ? 1u : 0u;
-            const float w = w_shape(__builtin_amdgcn_sqrtf(d2) * c.inv_h);
-            acc += (nb && j != s) ? q.w * w : 0.0f;
-        }
+        const uint32_t steps = wave_max(sl.count);
+        if (stats)
+            acc = density_bits<true>(bw, steps, lmask, lwj0, lane, spm, pi, c, acc, cnt);
+        else
+            acc = density_bits<false>(bw, steps, lmask, lwj0, lane, spm, pi, c, acc, cnt);
         seg_base += nw;
         wave_sync();
         if (cu.done) break;
     }
     if (lane == 0) gnw[item] = (uint32_t)seg_base;
-    if (active) reinterpret_cast<float*>(svr)[4 * s + 3] = pi.w * c.w0 + c.sigma * acc;
+    if (active) {
+        const float rho = c.sigma * acc;
+        float4 v = svr[s];
+        v.w = rho;
+        svr[s] = v;
+        // neighbour rows of the force walk: B = m/rho and C = B P / 2, so that
+        // m_j (P_i + P_j) / (2 rho_j) = (P_i / 2) B_j + C_j
+        const float B = pi.w / rho;
+        const float P = c.k_gas * (rho - c.rho0);
+        sfa[s] = make_float4(pi.x, pi.y, pi.z, B);
+        sfc[s] = make_float4(v.x, v.y, v.z, 0.5f * B * P);
+    }
     if (stats) add_nbr_stats(cnt, stats, lane);
 }
 
@@ -193,14 +222,71 @@ __device__ __forceinline__ float3 final_accel(float3 fp, float3 fv, float m_i, f
     return make_float3(fx * inv_m, (fy + c.grav * m) * inv_m, fz * inv_m);
 }
 
+// Pressure and viscosity sums of one target over neighbour rows (a = sfa[j], b = sfc[j]).
+// Pairs the filter let through beyond the radius (q >= 2), the self pair (r = 0, v_j = v_i)
+// and the padding slot (B = C = 0) all add exactly zero.
+struct ForceSums {
+    float3 fp = make_float3(0.f, 0.f, 0.f);
+    float3 fv = make_float3(0.f, 0.f, 0.f);
+    __device__ __forceinline__ void add(const float4& pi, const float3& vi, float p_half,
+                                        const float4& a, const float4& b, const StepConsts& c) {
+        const float rx = pi.x - a.x, ry = pi.y - a.y, rz = pi.z - a.z;
+        const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+        const float rinv = __builtin_amdgcn_rsqf(fmaxf(d2, 1e-30f));
+        const float rl = d2 * rinv;
+        const float q = rl * c.inv_h;
+        const float t2 = fmaxf(__builtin_fmaf(-c.inv_h, rl, 2.0f), 0.0f);
+        const bool inner = q <= 1.0f;
+        const float dwdq = inner ? q * __builtin_fmaf(1.5f, q, -2.0f) : -0.5f * t2 * t2;
+        const float lapf = inner ? __builtin_fmaf(3.0f, q, -2.0f) : t2;
+        // gradW is zero below |r| = 1e-6 (src/kernels.cpp:96-98)
+        const float coef = rl >= 1e-6f ? __builtin_fmaf(p_half, a.w, b.w) * dwdq * rinv : 0.0f;
+        fp.x -= coef * rx;
+        fp.y -= coef * ry;
+        fp.z -= coef * rz;
+        const float cv = a.w * lapf;
+        fv.x += cv * (b.x - vi.x);
+        fv.y += cv * (b.y - vi.y);
+        fv.z += cv * (b.z - vi.z);
+    }
+};
+
+// The wavefront runs its largest lane count; rows are requested one neighbour ahead, in two
+// alternating register sets.
+template <int kSeg>
+__device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
+                                           const uint32_t* __restrict__ lmask,
+                                           const uint32_t* __restrict__ lwj0, uint32_t lane,
+                                           const float4* __restrict__ sfa,
+                                           const float4* __restrict__ sfc, const float4& pi,
+                                           const float3& vi, float p_half, const StepConsts& c,
+                                           ForceSums& f) {
+    if (steps == 0) return;
+    uint32_t ja = bw.next(lmask, lwj0, lane);
+    float4 a0 = sfa[ja], b0 = sfc[ja];
+    uint32_t t = 0;
+    for (; t + 2 <= steps; t += 2) {
+        const uint32_t jb = bw.next(lmask, lwj0, lane);
+        const float4 a1 = sfa[jb], b1 = sfc[jb];
+        f.add(pi, vi, p_half, a0, b0, c);
+        ja = bw.next(lmask, lwj0, lane);
+        a0 = sfa[ja];
+        b0 = sfc[ja];
+        f.add(pi, vi, p_half, a1, b1, c);
+    }
+    if (t < steps) f.add(pi, vi, p_half, a0, b0, c);
+}
+
 template <class Shape>
 __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const float4* __restrict__ svr,
+    const float4* __restrict__ sfa, const float4* __restrict__ sfc,
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sid,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     const uint32_t* __restrict__ gmask, const uint32_t* __restrict__ gwj0,
-    const uint32_t* __restrict__ gnw, const float* __restrict__ dt_dev, float dt_arg,
+    const uint32_t* __restrict__ gnw, const uint32_t* __restrict__ ggrp,
+    const float* __restrict__ dt_dev, float dt_arg,
     float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc_out, float* __restrict__ rho_out,
     float* __restrict__ prs_out) {
@@ -217,13 +303,16 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     bool active;
     float4 pi;
     if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
-    const float4 vi = active ? svr[s] : make_float4(0.f, 0.f, 0.f, 1.f);
-    const float rho_i = vi.w;
+    init_sentinel<kSeg>(lmask, lwj0, lane, c);
+    const float4 vr = active ? svr[s] : make_float4(0.f, 0.f, 0.f, 1.f);
+    const float3 vi = make_float3(vr.x, vr.y, vr.z);
+    const float rho_i = vr.w;
     const float P_i = c.k_gas * (rho_i - c.rho0);
-    float3 fp = make_float3(0.f, 0.f, 0.f);
-    float3 fv = make_float3(0.f, 0.f, 0.f);
+    const float p_half = 0.5f * P_i;
+    ForceSums f;
     const int total_words = (int)uni(gnw[item]);
     const bool stored = total_words <= kMaxWords;
+    const uint32_t groups = stored ? uni(ggrp[item]) : 0u;
     Cursor cu;
     if (!stored) cursor_begin(cu, ch, cs, c);
     int seg_base = 0;
@@ -231,8 +320,8 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
         int nw;
         SegLane sl;
         if (stored) {
-            nw = min(kSeg, total_words - seg_base);
-            if (nw <= 0) break;
+            if (seg_base >= total_words) break;
+            nw = segment_end(seg_base, total_words, groups, kSeg - 1) - seg_base;
             const size_t base = (size_t)item * kMaxWords + seg_base;
             sl.nonempty = 0;
             sl.count = 0;
@@ -250,50 +339,16 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
         wave_sync();
         BitWalk<kSeg> bw;
         bw.ne = sl.nonempty;
-        // neighbour data is requested one iteration ahead; the look-ahead past the last
-        // neighbour re-reads the current slot
-        uint32_t jn = sl.count ? bw.next(lmask, lwj0, lane) : 0u;
-        float4 pn = spm[jn];
-        float4 vn = svr[jn];
-        for (uint32_t t = 0; t < sl.count; ++t) {
-            const uint32_t j = jn;
-            const float4 pj = pn;
-            const float4 vj = vn;
-            if (t + 1 < sl.count) jn = bw.next(lmask, lwj0, lane);
-            pn = spm[jn];
-            vn = svr[jn];
-            const float d2 = dist2_exact(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
-            const float rl = __builtin_amdgcn_sqrtf(d2);
-            const float q = rl * c.inv_h;
-            const float t2 = fmaxf(2.0f - q, 0.0f);
-            const float inv_rho_j = __builtin_amdgcn_rcpf(vj.w);
-            const bool inner = q <= 1.0f;
-            const float dwdq = inner ? q * (-2.0f + 1.5f * q) : -0.5f * t2 * t2;
-            const float lapf = inner ? (-2.0f + 3.0f * q) : t2;
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const bool other = j != s && d2 <= c.r2;  // the reference's test, self excluded
-            const float P_j = c.k_gas * (vj.w - c.rho0);
-            // pressure: skipped for |r| < 1e-6 (and self); viscosity: skipped for self only
-            const float coef = (other && rl >= 1e-6f)
-                                   ? pj.w * (P_i + P_j) * 0.5f * inv_rho_j * dwdq *
-                                         __builtin_amdgcn_rcpf(rl)
-                                   : 0.0f;
-            fp.x -= coef * rx;
-            fp.y -= coef * ry;
-            fp.z -= coef * rz;
-            const float cv = other ? pj.w * inv_rho_j * lapf : 0.0f;
-            fv.x += cv * (vj.x - vi.x);
-            fv.y += cv * (vj.y - vi.y);
-            fv.z += cv * (vj.z - vi.z);
-        }
+        force_bits<kSeg>(bw, wave_max(sl.count), lmask, lwj0, lane, sfa, sfc, pi, vi, p_half, c,
+                         f);
         seg_base += nw;
         wave_sync();
         if (!stored && cu.done) break;
     }
     if (!active) return;
-    const float3 a = final_accel(fp, fv, pi.w, rho_i, c);
+    const float3 a = final_accel(f.fp, f.fv, pi.w, rho_i, c);
     const float dt = dt_dev ? *dt_dev : dt_arg;
-    float3 v = make_float3(vi.x, vi.y, vi.z);
+    float3 v = vi;
     float4 p = pi;
     const SceneGrid& g = scenes[ch.scene];
     integrate_clamp(p, v, a, dt, c, g.lo, g.hi);
@@ -362,7 +417,7 @@ __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
                     lane, ch, s, active, pi))
         return;
     Cursor cu;
-    cursor_begin(cu, ch, cs, c);
+    cursor_begin(cu, ch, cs, c, 1);
     uint32_t cnt = 0;
     int64_t o = (pass == 1 && active) ? offs_by_id[sid[s]] : 0;
     for (;;) {
@@ -426,20 +481,23 @@ static uint32_t grid_for(uint32_t chunks) {
     }
 
 hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, float4* svr,
-                               const uint32_t* skey, const uint32_t* cell_start, uint32_t n,
+                               float4* sfa, float4* sfc, const uint32_t* skey,
+                               const uint32_t* cell_start, uint32_t n,
                                const StepConsts& c, const SceneGrid* scenes,
                                unsigned long long* nbr_stats, int shape, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const uint32_t chunks = max_chunks(n, c);
     SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_density_walk<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, svr, skey, cell_start, c, scenes, wb.mask,
-                                                  wb.wj0, wb.nwords, nbr_stats));
+                                                  spm, svr, sfa, sfc, skey, cell_start, c,
+                                                  scenes, wb.mask, wb.wj0, wb.nwords,
+                                                  wb.groups, nbr_stats));
     return hipGetLastError();
 }
 
 hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const float4* svr,
-                             const uint32_t* skey, const uint32_t* sid,
+                             const float4* sfa, const float4* sfc, const uint32_t* skey,
+                             const uint32_t* sid,
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
                              const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
                              float4* vel, float4* acc, float* rho, float* prs, int shape,
@@ -448,8 +506,10 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const flo
     const uint32_t chunks = max_chunks(n, c);
     SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_force_walk<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, svr, skey, sid, cell_start, c, scenes,
-                                                  wb.mask, wb.wj0, wb.nwords, dt_dev, dt, pm, vel,
+                                                  spm, svr, sfa, sfc, skey, sid, cell_start, c,
+                                                  scenes,
+                                                  wb.mask, wb.wj0, wb.nwords, wb.groups, dt_dev,
+                                                  dt, pm, vel,
                                                   acc, rho, prs));
     // the non-finite bin holds at most n particles; the kernel reads its start on the device
     hipLaunchKernelGGL(k_nonfinite_step, dim3((n + 255) / 256), dim3(256), 0, st, spm,

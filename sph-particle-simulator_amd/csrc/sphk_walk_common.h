@@ -153,11 +153,34 @@ struct Chunk {
     int zlo, zhi;         // candidate z cells (inclusive), grid-relative
 };
 
-// Cursor over the candidate columns, in (dx, dy) lexicographic order.
+// Candidate columns in walk order, as (dx + 2) * 5 + (dy + 2).  Row 0, balanced order: the
+// four corner columns, the four edge columns, the centre, then the ring two columns out
+// (walked only when axis_edge_flags asks for it).  A lane near one side of the chunk's
+// column finds about as many neighbours in each of the first three groups as a lane near
+// the other side, so segments that hold whole groups have even per-lane bit counts (the
+// wavefront runs the largest).  Row 1: the reference's (dx, dy) lexicographic order
+// (SpatialHash::query_squared), for the neighbour-list export.
+__constant__ uint8_t kColumnOrder[2][25] = {
+    {6, 18, 8, 16, 7, 17, 11, 13, 12, 0, 1, 2, 3, 4, 5, 9, 10, 14, 15, 19, 20, 21, 22, 23, 24},
+    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24}};
+
+// Balance group of position k of the balanced order.
+__device__ __forceinline__ int column_group(int k) { return k < 4 ? 0 : k < 8 ? 1 : k < 9 ? 2 : 3; }
+
+// Cursor over the candidate columns of a chunk.
 struct Cursor {
-    int dx, dy;
+    int k;            // position in the column order
+    int order;        // 0 balanced, 1 reference
+    int dx, dy;       // current column
+    int group;        // balance group of the current column (0 in the reference order)
+    bool fresh;       // the current column opens its group
     uint32_t j, end;  // current slot and end of the current column range
     bool done;
+    int gw0, gw1, gw2, gw3;  // words (32 candidates, never straddling columns) of each group
+    // words of the current group (separate fields and selects keep the cursor in registers)
+    __device__ __forceinline__ int group_words() const {
+        return group == 0 ? gw0 : group == 1 ? gw1 : group == 2 ? gw2 : gw3;
+    }
 };
 
 __device__ __forceinline__ void column_range(const Chunk& ch, int dx, int dy,
@@ -169,31 +192,87 @@ __device__ __forceinline__ void column_range(const Chunk& ch, int dx, int dy,
     b = uni(cs[base + (uint32_t)ch.zhi + 1u]);
 }
 
+// Column at order position k, if the chunk walks it.
+__device__ __forceinline__ bool column_at(const Chunk& ch, int order, int k, int& dx, int& dy) {
+    const int code = (int)kColumnOrder[order][k];
+    dx = code / 5 - 2;
+    dy = code % 5 - 2;
+    return dx >= ch.dx0 && dx <= ch.dx1 && dy >= ch.dy0 && dy <= ch.dy1;
+}
+
+// Moves to the first walked column at or after position k (done when none is left).
+__device__ __forceinline__ void cursor_seek(Cursor& cu, const Chunk& ch,
+                                            const uint32_t* __restrict__ cs, const StepConsts& c,
+                                            int k) {
+    for (; k < 25; ++k) {
+        int dx, dy;
+        if (!column_at(ch, cu.order, k, dx, dy)) continue;
+        const int g = cu.order == 0 ? column_group(k) : 0;
+        cu.fresh = k == 0 || g != cu.group;
+        cu.k = k;
+        cu.dx = dx;
+        cu.dy = dy;
+        cu.group = g;
+        column_range(ch, dx, dy, cs, c, cu.j, cu.end);
+        return;
+    }
+    cu.k = 25;
+    cu.done = true;
+}
+
+// order 0: balanced (the SPH walks); 1: the reference's (neighbour-list export).
 __device__ __forceinline__ void cursor_begin(Cursor& cu, const Chunk& ch,
-                                             const uint32_t* __restrict__ cs, const StepConsts& c) {
-    cu.dx = ch.dx0;
-    cu.dy = ch.dy0;
+                                             const uint32_t* __restrict__ cs, const StepConsts& c,
+                                             int order = 0) {
+    cu.order = order;
     cu.done = false;
-    column_range(ch, cu.dx, cu.dy, cs, c, cu.j, cu.end);
+    cu.group = -1;
+    cu.gw0 = cu.gw1 = cu.gw2 = cu.gw3 = 0;
+    if (order == 0) {
+        for (int k = 0; k < 25; ++k) {
+            int dx, dy;
+            if (!column_at(ch, 0, k, dx, dy)) continue;
+            uint32_t a, b;
+            column_range(ch, dx, dy, cs, c, a, b);
+            const int w = (int)((b - a + 31u) / 32u);
+            const int g = column_group(k);
+            cu.gw0 += g == 0 ? w : 0;
+            cu.gw1 += g == 1 ? w : 0;
+            cu.gw2 += g == 2 ? w : 0;
+            cu.gw3 += g == 3 ? w : 0;
+        }
+    }
+    cursor_seek(cu, ch, cs, c, 0);
 }
 
 // Moves to the next column when the current one is exhausted; false when all are.
 __device__ __forceinline__ bool cursor_settle(Cursor& cu, const Chunk& ch,
                                               const uint32_t* __restrict__ cs,
                                               const StepConsts& c) {
-    while (cu.j >= cu.end) {
-        if (cu.dy < ch.dy1) {
-            ++cu.dy;
-        } else if (cu.dx < ch.dx1) {
-            ++cu.dx;
-            cu.dy = ch.dy0;
-        } else {
-            cu.done = true;
-            return false;
-        }
-        column_range(ch, cu.dx, cu.dy, cs, c, cu.j, cu.end);
+    while (!cu.done && cu.j >= cu.end) cursor_seek(cu, ch, cs, c, cu.k + 1);
+    return !cu.done;
+}
+
+// Group starts of a chunk's stored words, packed 8 bits each: words before group 1, 2, 3.
+__device__ __forceinline__ uint32_t pack_groups(const Cursor& cu) {
+    const uint32_t b1 = (uint32_t)cu.gw0;
+    const uint32_t b2 = b1 + (uint32_t)cu.gw1;
+    const uint32_t b3 = b2 + (uint32_t)cu.gw2;
+    return b1 | b2 << 8 | b3 << 16;
+}
+
+// End of the stored-word segment that starts at word `pos`: the last group start (or the
+// end) within `cap` words, or pos + cap when the group at pos is larger than that.
+__device__ __forceinline__ int segment_end(int pos, int total, uint32_t groups, int cap) {
+    const int lim = min(pos + cap, total);
+    if (lim == total) return total;
+    int end = pos + cap;
+#pragma unroll
+    for (int g = 2; g >= 0; --g) {
+        const int b = (int)((groups >> (8 * g)) & 0xffu);
+        if (b > pos && b <= lim) return b;
     }
-    return true;
+    return end;
 }
 
 // Appends one test result to a 32-bit word: word = 2*word + (d2 <= r2).  One compare into
@@ -214,9 +293,10 @@ struct SegLane {
     uint32_t count;
 };
 
-// Phase 1: fills up to kSeg words of filter bits into LDS (masks[w*64 + lane], wj0[w]): a
-// superset of the neighbours (see dist2_filter).  Candidate k of a word sits in bit 31-k
-// (words of fewer than 32 candidates are left-aligned).  Returns the number of words written.
+// Phase 1: fills up to kSeg - 1 words of filter bits into LDS (masks[w*64 + lane], wj0[w]):
+// a superset of the neighbours (see dist2_filter).  Candidate k of a word sits in bit 31-k
+// (words of fewer than 32 candidates are left-aligned).  Word kSeg - 1 is the sentinel of
+// BitWalk and is never written here.  Returns the number of words written.
 template <int kSeg>
 __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
                                               const float4* __restrict__ spm,
@@ -229,7 +309,12 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
     int w = 0;
     sl.nonempty = 0;
     sl.count = 0;
-    while (w < kSeg && cursor_settle(cu, ch, cs, c)) {
+    while (w < kSeg - 1 && cursor_settle(cu, ch, cs, c)) {
+        if (cu.fresh) {
+            // a group that does not fit in what is left of the segment starts the next one
+            if (w > 0 && w + cu.group_words() > kSeg - 1) break;
+            cu.fresh = false;
+        }
         const uint32_t j0 = uni(cu.j);
         const uint32_t cnt = uni(min(32u, cu.end - j0));
         uint32_t word = 0;
@@ -303,9 +388,11 @@ __device__ __forceinline__ bool load_chunk(uint32_t item, const uint32_t* __rest
     return true;
 }
 
-// Per-lane walk over the set bits of one segment, in candidate order.  Run exactly
-// SegLane::count times; refills from the next non-empty word by selects (no inner loop).
-// The word index read ahead stays inside the segment (kSeg - 1 when nothing is left).
+// Per-lane walk over the set bits of one segment, in candidate order; refills from the next
+// non-empty word by selects (no inner loop).  Word kSeg - 1 of the segment is a sentinel
+// (bit 31 in every lane, first slot = StepConsts::pad_slot, see init_sentinel): once a
+// lane's bits are used up it returns the padding slot on every call, so a wavefront can run
+// its largest lane count with no per-lane condition.
 template <int kSeg>
 struct BitWalk {
     uint32_t m = 0, jb = 0, ne = 0;
@@ -323,6 +410,16 @@ struct BitWalk {
         return jb + k;
     }
 };
+
+// Writes the sentinel word kSeg - 1 of a wavefront's segment buffers (once per wavefront;
+// phase 1 and the stored-word loads never overwrite it).
+template <int kSeg>
+__device__ __forceinline__ void init_sentinel(uint32_t* __restrict__ lmask,
+                                              uint32_t* __restrict__ lwj0, uint32_t lane,
+                                              const StepConsts& c) {
+    lmask[(kSeg - 1) * 64 + lane] = 0x80000000u;
+    if (lane == 0) lwj0[kSeg - 1] = c.pad_slot;
+}
 
 __device__ __forceinline__ void add_nbr_stats(uint32_t cnt, unsigned long long* stats,
                                               uint32_t lane) {
