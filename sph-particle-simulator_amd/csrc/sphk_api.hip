@@ -91,6 +91,13 @@ StepConsts make_consts(const sphk_params& p) {
     c.grav = p.gravity;
     c.damping = p.damping;
     c.formulation = p.formulation;
+    // Quantized candidate filter.  X = floor(x * qscale) per axis (FP64) differs from
+    // (x_a - x_b) qscale by under 1 + 2^-12, so a pair that passes the FP32 test (true
+    // distance below sqrt(r2) (1 + 3 * 2^-24)) has |dX| < bound and dX^2 + dY^2 + dZ^2 <= T.
+    c.qscale = 256.0 * (double)c.inv_cell;
+    const double bound =
+        std::sqrt((double)c.r2) * (1.0 + 1e-6) * c.qscale + std::sqrt(3.0) * 1.001;
+    c.qneg = -(int32_t)(std::floor(bound * bound) + 1.0);
     return c;
 }
 
@@ -376,7 +383,7 @@ int build_sorted(sphk_engine* e) {
     e->skey = ko;
     e->sid = vo;
     HIP_TRY(launch_cell_start(e->skey, n, e->consts.G, e->cell_start, st));
-    HIP_TRY(launch_gather(e->pm, e->vel, e->sid, n, e->spm, e->svr, st));
+    HIP_TRY(launch_gather(e->pm, e->vel, e->sid, n, e->consts.qscale, e->sq, e->spm, e->svr, st));
     HIP_TRY(launch_chunks(e->cell_start, e->consts, e->d_scenes, e->walk.colcnt, e->scan_scratch,
                           e->walk.items, e->walk.nitems, st));
     e->sorted_valid = true;
@@ -480,10 +487,10 @@ int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev) {
     if (adaptive) HIP_TRY(launch_cfl(e->vel, e->acc, n, e->params, e->uscal + 1, e->fscal, st));
     if (int rc = build_sorted(e)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
-    HIP_TRY(launch_density_walk(e->walk, e->spm, e->svr, e->sfa, e->sfc, e->skey, e->cell_start,
+    HIP_TRY(launch_density_walk(e->walk, e->spm, e->sq, e->svr, e->sfa, e->sfc, e->skey, e->cell_start,
                                 n, e->consts, e->d_scenes, e->nstats, e->shape_density, st));
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
-    HIP_TRY(launch_force_walk(e->walk, e->spm, e->svr, e->sfa, e->sfc, e->skey, e->sid,
+    HIP_TRY(launch_force_walk(e->walk, e->spm, e->sq, e->svr, e->sfa, e->sfc, e->skey, e->sid,
                               e->cell_start, n,
                               e->consts, e->d_scenes, adaptive ? e->fscal : nullptr, dt, e->pm,
                               e->vel, e->acc, e->rho, e->prs, e->shape_force, st));
@@ -571,6 +578,7 @@ int sphk_create(size_t capacity, sphk_engine** out) {
     HIP_TRY(dalloc(&e->svr, C + 1));
     HIP_TRY(dalloc(&e->sfa, C + 1));
     HIP_TRY(dalloc(&e->sfc, C + 1));
+    HIP_TRY(dalloc(&e->sq, C + 1));
     {
         // padding slot: zero mass and weights, 1e15 away from any particle
         const float4 far_row = make_float4(1e15f, 1e15f, 1e15f, 0.0f);
@@ -602,7 +610,7 @@ int sphk_destroy(sphk_engine* e) {
     void* ptrs[] = {e->pm,   e->vel,      e->acc,  e->rho,      e->prs,  e->snap_pm,
                     e->snap_vel, e->snap_acc, e->keys, e->keys_alt, e->vals, e->vals_alt,
                     e->hist, e->spm,      e->svr,  e->cell_start, e->fscal, e->uscal,
-                    e->nstats, e->sfa, e->sfc};
+                    e->nstats, e->sfa, e->sfc, e->sq};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : {(void*)e->walk.items, (void*)e->walk.nitems, (void*)e->walk.colcnt,
@@ -1116,7 +1124,7 @@ int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t 
     uint32_t* dcnt = nullptr;
     HIP_TRY(dalloc(&dcnt, n));
     HIP_TRY(launch_fill_u32(dcnt, 0u, n, e->stream));
-    HIP_TRY(launch_nbr_lists(e->walk, e->spm, e->skey, e->sid, e->cell_start, (uint32_t)n,
+    HIP_TRY(launch_nbr_lists(e->walk, e->spm, e->sq, e->skey, e->sid, e->cell_start, (uint32_t)n,
                              e->consts, e->d_scenes, 0, dcnt, nullptr, nullptr, e->shape_density,
                              e->stream));
     std::vector<uint32_t> cnt(n);
@@ -1139,7 +1147,7 @@ int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t 
     HIP_TRY(dalloc(&doffs, n + 1));
     HIP_TRY(dalloc(&didx, (size_t)o));
     HIP_TRY(hipMemcpyAsync(doffs, offs.data(), (n + 1) * 8, hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(launch_nbr_lists(e->walk, e->spm, e->skey, e->sid, e->cell_start, (uint32_t)n,
+    HIP_TRY(launch_nbr_lists(e->walk, e->spm, e->sq, e->skey, e->sid, e->cell_start, (uint32_t)n,
                              e->consts, e->d_scenes, 1, nullptr, doffs, didx, e->shape_density,
                              e->stream));
     if (o) HIP_TRY(hipMemcpyAsync(idx, didx, (size_t)o * 8, hipMemcpyDeviceToHost, e->stream));

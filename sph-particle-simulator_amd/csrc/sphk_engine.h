@@ -101,6 +101,7 @@ struct sphk_engine {
     float4* svr = nullptr;
     float4* sfa = nullptr;
     float4* sfc = nullptr;
+    uint2* sq = nullptr;  // quantized coordinates of the candidate filter (x|y 16 bits, z 32)
     uint32_t* cell_start = nullptr;
     uint64_t cell_start_cap = 0;
     uint32_t* skey = nullptr;  // -> keys or keys_alt after the sort

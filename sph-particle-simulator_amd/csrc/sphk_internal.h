@@ -38,6 +38,8 @@ struct StepConsts {
     uint32_t nscenes;
     uint32_t pad_slot;  // sorted slot past every particle (index = capacity): zero mass and
                         // weights, far away; the walks read it in place of a neighbour
+    int32_t qneg;       // -(T + 1), T the threshold of the quantized candidate filter
+    double qscale;      // quantized coordinate = floor(x * qscale): 256 units per cell
 };
 
 // Dense cell grid of one scene (z fastest) and its place in the global cell/column
@@ -125,24 +127,28 @@ size_t radix_hist_words(uint32_t n, int total_bits);
 hipError_t launch_cell_start(const uint32_t* skey, uint32_t n, uint32_t G, uint32_t* cell_start,
                              hipStream_t st);
 hipError_t launch_gather(const float4* pm, const float4* vel, const uint32_t* sid, uint32_t n,
+                         double qscale, uint2* sq,
                          float4* spm, float4* svr, hipStream_t st);
 uint32_t max_chunks(uint32_t n, const StepConsts& c);
 hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, const SceneGrid* scenes,
                          uint32_t* cnt_scratch, uint32_t* scan_scratch, uint32_t* items,
                          uint32_t* nitems, hipStream_t st);
-hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, float4* svr,
+hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
+                               float4* svr,
                                float4* sfa, float4* sfc, const uint32_t* skey,
                                const uint32_t* cell_start, uint32_t n,
                                const StepConsts& c, const SceneGrid* scenes,
                                unsigned long long* nbr_stats, int shape, hipStream_t st);
-hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const float4* svr,
+hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
+                             const float4* svr,
                              const float4* sfa, const float4* sfc, const uint32_t* skey,
                              const uint32_t* sid,
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
                              const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
                              float4* vel, float4* acc, float* rho, float* prs, int shape,
                              hipStream_t st);
-hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint32_t* skey,
+hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint2* sq,
+                            const uint32_t* skey,
                             const uint32_t* sid, const uint32_t* cell_start, uint32_t n,
                             const StepConsts& c, const SceneGrid* scenes, int pass,
                             uint32_t* count_by_id, const int64_t* offsets_by_id, int64_t* idx,

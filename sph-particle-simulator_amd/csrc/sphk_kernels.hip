@@ -307,22 +307,38 @@ hipError_t launch_cell_start(const uint32_t* skey, uint32_t n, uint32_t G, uint3
 // ---------------------------------------------------------------------------
 // gather into cell-sorted order
 // ---------------------------------------------------------------------------
+// Quantized coordinate of the candidate filter: floor(x * qscale) modulo 2^32, computed in
+// FP64 (exact for |x * qscale| < 2^52; 0 for non-finite or larger values, whose particles
+// are never candidates).
+__device__ __forceinline__ uint32_t quantize(float x, double qscale) {
+    const double v = floor((double)x * qscale);
+    if (!(fabs(v) < 4503599627370496.0)) return 0u;
+    return (uint32_t)(v - 4294967296.0 * floor(v * (1.0 / 4294967296.0)));
+}
+
 __global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ pm,
                                                 const float4* __restrict__ vel,
                                                 const uint32_t* __restrict__ sid, uint32_t n,
+                                                double qscale, uint2* __restrict__ sq,
                                                 float4* __restrict__ spm, float4* __restrict__ svr) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     uint32_t id = sid[s];
-    spm[s] = pm[id];
+    const float4 p = pm[id];
+    spm[s] = p;
     float4 v = vel[id];
     svr[s] = make_float4(v.x, v.y, v.z, 0.0f);
+    // x and y keep their low 16 bits (packed), z its low 32: differences of neighbours are
+    // exact in 16- and 24-bit wrap-around arithmetic
+    const uint32_t qx = quantize(p.x, qscale), qy = quantize(p.y, qscale);
+    sq[s] = make_uint2((qx & 0xffffu) | (qy << 16), quantize(p.z, qscale));
 }
 
 hipError_t launch_gather(const float4* pm, const float4* vel, const uint32_t* sid, uint32_t n,
-                         float4* spm, float4* svr, hipStream_t st) {
+                         double qscale, uint2* sq, float4* spm, float4* svr, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, pm, vel, sid, n, spm, svr);
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, pm, vel, sid, n, qscale,
+                       sq, spm, svr);
     return hipGetLastError();
 }
 

@@ -104,7 +104,8 @@ __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
 template <class Shape>
 __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
-    const float4* __restrict__ spm, float4* __restrict__ svr, float4* __restrict__ sfa,
+    const float4* __restrict__ spm, const uint2* __restrict__ sq, float4* __restrict__ svr,
+    float4* __restrict__ sfa,
     float4* __restrict__ sfc, const uint32_t* __restrict__ skey,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     uint32_t* __restrict__ gmask, uint32_t* __restrict__ gwj0, uint32_t* __restrict__ gnw,
@@ -122,6 +123,7 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     bool active;
     float4 pi;
     if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
+    const uint2 qi = active ? sq[s] : make_uint2(0u, 0u);
     init_sentinel<kSeg>(lmask, lwj0, lane, c);
     Cursor cu;
     cursor_begin(cu, ch, cs, c);
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     for (;;) {
         SegLane sl;
         const int nw =
-            phase1_segment<kSeg>(cu, ch, spm, cs, c, pi.x, pi.y, pi.z, lmask, lwj0, lane, sl);
+            phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane, sl);
         if (nw == 0) break;
         wave_sync();
         store_words(item, seg_base, nw, lmask, lwj0, gmask, gwj0, lane);
@@ -280,8 +282,9 @@ __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
 template <class Shape>
 __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
-    const float4* __restrict__ spm, const float4* __restrict__ svr,
-    const float4* __restrict__ sfa, const float4* __restrict__ sfc,
+    const float4* __restrict__ spm, const uint2* __restrict__ sq,
+    const float4* __restrict__ svr, const float4* __restrict__ sfa,
+    const float4* __restrict__ sfc,
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sid,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     const uint32_t* __restrict__ gmask, const uint32_t* __restrict__ gwj0,
@@ -303,6 +306,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     bool active;
     float4 pi;
     if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
+    const uint2 qi = active ? sq[s] : make_uint2(0u, 0u);
     init_sentinel<kSeg>(lmask, lwj0, lane, c);
     const float4 vr = active ? svr[s] : make_float4(0.f, 0.f, 0.f, 1.f);
     const float3 vi = make_float3(vr.x, vr.y, vr.z);
@@ -333,7 +337,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
             }
             for (int w = (int)lane; w < nw; w += 64) lwj0[w] = gwj0[base + w];
         } else {
-            nw = phase1_segment<kSeg>(cu, ch, spm, cs, c, pi.x, pi.y, pi.z, lmask, lwj0, lane, sl);
+            nw = phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane, sl);
             if (nw == 0) break;
         }
         wave_sync();
@@ -397,9 +401,10 @@ __global__ __launch_bounds__(256) void k_nonfinite_step(
 template <class Shape>
 __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
-    const float4* __restrict__ spm, const uint32_t* __restrict__ skey,
-    const uint32_t* __restrict__ sid, const uint32_t* __restrict__ cs, StepConsts c,
-    const SceneGrid* __restrict__ scenes, int pass,
+    const float4* __restrict__ spm, const uint2* __restrict__ sq,
+    const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sid,
+    const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
+    int pass,
     uint32_t* __restrict__ cnt_by_id, const int64_t* __restrict__ offs_by_id,
     int64_t* __restrict__ idx) {
     constexpr int kSeg = Shape::seg;
@@ -416,6 +421,7 @@ __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
     if (!load_chunk(swizzled_block() * Shape::waves + wave, items, nitems, skey, cs, spm, c, scenes,
                     lane, ch, s, active, pi))
         return;
+    const uint2 qi = active ? sq[s] : make_uint2(0u, 0u);
     Cursor cu;
     cursor_begin(cu, ch, cs, c, 1);
     uint32_t cnt = 0;
@@ -423,7 +429,7 @@ __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
     for (;;) {
         SegLane sl;
         const int nw =
-            phase1_segment<kSeg>(cu, ch, spm, cs, c, pi.x, pi.y, pi.z, lmask, lwj0, lane, sl);
+            phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane, sl);
         if (nw == 0) break;
         wave_sync();
         BitWalk<kSeg> bw;
@@ -480,7 +486,8 @@ static uint32_t grid_for(uint32_t chunks) {
         default: { using S = Shape4; CALL; break; } \
     }
 
-hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, float4* svr,
+hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
+                               float4* svr,
                                float4* sfa, float4* sfc, const uint32_t* skey,
                                const uint32_t* cell_start, uint32_t n,
                                const StepConsts& c, const SceneGrid* scenes,
@@ -489,13 +496,14 @@ hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, float4*
     const uint32_t chunks = max_chunks(n, c);
     SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_density_walk<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, svr, sfa, sfc, skey, cell_start, c,
+                                                  spm, sq, svr, sfa, sfc, skey, cell_start, c,
                                                   scenes, wb.mask, wb.wj0, wb.nwords,
                                                   wb.groups, nbr_stats));
     return hipGetLastError();
 }
 
-hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const float4* svr,
+hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
+                             const float4* svr,
                              const float4* sfa, const float4* sfc, const uint32_t* skey,
                              const uint32_t* sid,
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
@@ -506,7 +514,8 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const flo
     const uint32_t chunks = max_chunks(n, c);
     SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_force_walk<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, svr, sfa, sfc, skey, sid, cell_start, c,
+                                                  spm, sq, svr, sfa, sfc, skey, sid,
+                                                  cell_start, c,
                                                   scenes,
                                                   wb.mask, wb.wj0, wb.nwords, wb.groups, dt_dev,
                                                   dt, pm, vel,
@@ -518,7 +527,8 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const flo
     return hipGetLastError();
 }
 
-hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint32_t* skey,
+hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint2* sq,
+                            const uint32_t* skey,
                             const uint32_t* sid, const uint32_t* cell_start, uint32_t n,
                             const StepConsts& c, const SceneGrid* scenes, int pass,
                             uint32_t* count_by_id, const int64_t* offsets_by_id, int64_t* idx,
@@ -527,8 +537,8 @@ hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint
     const uint32_t chunks = max_chunks(n, c);
     SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_nbr_lists<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, skey, sid, cell_start, c, scenes, pass,
-                                                  count_by_id, offsets_by_id, idx));
+                                                  spm, sq, skey, sid, cell_start, c, scenes,
+                                                  pass, count_by_id, offsets_by_id, idx));
     return hipGetLastError();
 }
 

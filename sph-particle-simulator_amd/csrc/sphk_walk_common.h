@@ -293,15 +293,31 @@ struct SegLane {
     uint32_t count;
 };
 
+// Appends the quantized filter's verdict on one candidate to a 32-bit word: with
+// dX, dY (16-bit wrap-around) and dZ (24-bit) the differences of the quantized coordinates,
+// t = dX^2 + dY^2 + dZ^2 + qneg is negative exactly when the squared distance is at most T,
+// and word = 2 * word + sign(t).  Five VALU: packed 16-bit subtract, subtract, 24-bit
+// multiply-add, dot2, align-bit.  Wrap-around can only turn a far candidate into a passing
+// one, never a neighbour into a failing one.
+typedef short sphk_short2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t push_quant(uint32_t word, uint2 a, uint2 b, int32_t qneg) {
+    const sphk_short2 dxy =
+        __builtin_bit_cast(sphk_short2, a.x) - __builtin_bit_cast(sphk_short2, b.x);
+    const uint32_t dz = (uint32_t)(((int32_t)((a.y - b.y) << 8)) >> 8);
+    const int32_t t = __builtin_amdgcn_sdot2(dxy, dxy, (int32_t)(dz * dz + (uint32_t)qneg), false);
+    return __builtin_amdgcn_alignbit(word, (uint32_t)t, 31);
+}
+
 // Phase 1: fills up to kSeg - 1 words of filter bits into LDS (masks[w*64 + lane], wj0[w]):
-// a superset of the neighbours (see dist2_filter).  Candidate k of a word sits in bit 31-k
-// (words of fewer than 32 candidates are left-aligned).  Word kSeg - 1 is the sentinel of
-// BitWalk and is never written here.  Returns the number of words written.
+// a superset of the neighbours (see push_quant; lanes past the chunk get empty words).
+// Candidate k of a word sits in bit 31-k (words of fewer than 32 candidates are
+// left-aligned).  Word kSeg - 1 is the sentinel of BitWalk and is never written here.
+// Returns the number of words written.
 template <int kSeg>
 __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
-                                              const float4* __restrict__ spm,
+                                              const uint2* __restrict__ sq,
                                               const uint32_t* __restrict__ cs,
-                                              const StepConsts& c, float px, float py, float pz,
+                                              const StepConsts& c, uint2 qi, bool active,
                                               uint32_t* __restrict__ lmask,
                                               uint32_t* __restrict__ lwj0, uint32_t lane,
                                               SegLane& sl) {
@@ -318,20 +334,15 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
         const uint32_t j0 = uni(cu.j);
         const uint32_t cnt = uni(min(32u, cu.end - j0));
         uint32_t word = 0;
-        const float4* __restrict__ pb = spm + j0;  // uniform base: immediate offsets below
+        const uint2* __restrict__ pb = sq + j0;  // uniform base: scalar loads
         if (cnt == 32u) {
 #pragma unroll
-            for (int k = 0; k < 32; ++k) {
-                const float4 q = pb[k];
-                word = push_bit(word, dist2_filter(px, py, pz, q.x, q.y, q.z), c.r2_filter);
-            }
+            for (int k = 0; k < 32; ++k) word = push_quant(word, qi, pb[k], c.qneg);
         } else {
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4 q = pb[k];
-                word = push_bit(word, dist2_filter(px, py, pz, q.x, q.y, q.z), c.r2_filter);
-            }
+            for (uint32_t k = 0; k < cnt; ++k) word = push_quant(word, qi, pb[k], c.qneg);
             word <<= (32u - cnt);
         }
+        word = active ? word : 0u;
         lmask[w * 64 + lane] = word;
         if (lane == 0) lwj0[w] = j0;
         sl.nonempty |= (word != 0u ? 1u : 0u) << w;

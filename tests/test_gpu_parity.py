@@ -153,6 +153,49 @@ def test_random_points_against_brute_force(sphk, oracle):
         assert sorted(idx[offs[i]:offs[i + 1]].tolist()) == bidx[boffs[i]:boffs[i + 1]].tolist()
 
 
+def _boundary_partner(b, u, r2):
+    """FP32 point p = b + t u with the largest t (bisection on the FP32 test) such that
+    (dx^2 + dy^2) + dz^2 <= r2 in FP32: a pair on the very edge of the radius."""
+    def inside(t):
+        p = (b.astype(np.float64) + t * u).astype(np.float32)
+        d = (b - p).astype(np.float32)
+        d2 = np.float32(np.float32(d[0] * d[0]) + np.float32(d[1] * d[1])) + np.float32(d[2] * d[2])
+        return d2 <= r2, p
+    lo, hi = 0.0, 0.05
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if inside(mid)[0]:
+            lo = mid
+        else:
+            hi = mid
+    return inside(lo)[1]
+
+
+def test_edge_pairs_in_all_directions_against_brute_force(sphk, oracle):
+    # Pairs exactly on the FP32 edge of the radius, in random directions and at coordinates
+    # of several magnitudes: the candidate filter may pass extra pairs but must keep all of
+    # these (the lists must equal the brute-force FP32 result).
+    rng = np.random.default_rng(31)
+    r2 = np.float32(R2)
+    pts = []
+    for centre in ((0.0, 0.0, 0.0), (3.7, -2.9, 1.3), (-5.1, 4.2, -3.3), (1e-3, 2.5, -4.9)):
+        for _ in range(150):
+            b = (np.array(centre) + rng.uniform(-0.3, 0.3, 3)).astype(np.float32)
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            pts += [b, _boundary_partner(b, u, r2)]
+    pos = np.array(pts, np.float32)
+    e = sphk.Engine(len(pos), sphk.default_params(xmin=-8, xmax=8, ymin=-8, ymax=8, zmin=-8,
+                                                   zmax=8))
+    e.upload(pos)
+    offs, idx = e.neighbors()
+    boffs, bidx = oracle.brute_neighbors(pos, r2)
+    for i in range(len(pos)):
+        assert sorted(idx[offs[i]:offs[i + 1]].tolist()) == bidx[boffs[i]:boffs[i + 1]].tolist()
+    for k in range(0, len(pos), 2):
+        assert k + 1 in idx[offs[k]:offs[k + 1]].tolist()
+
+
 def test_nonfinite_particles(sphk, oracle):
     pos = np.array([[0.1, 0.1, 0.1], [np.nan, 0.1, 0.1], [0.11, 0.1, 0.1], [0.1, np.inf, 0.1],
                     [0.12, 0.1, 0.1]], np.float32)
