@@ -487,10 +487,10 @@ int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev) {
     if (adaptive) HIP_TRY(launch_cfl(e->vel, e->acc, n, e->params, e->uscal + 1, e->fscal, st));
     if (int rc = build_sorted(e)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
-    HIP_TRY(launch_density_walk(e->walk, e->spm, e->sq, e->svr, e->sfa, e->sfc, e->skey, e->cell_start,
+    HIP_TRY(launch_density_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->cell_start,
                                 n, e->consts, e->d_scenes, e->nstats, e->shape_density, st));
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
-    HIP_TRY(launch_force_walk(e->walk, e->spm, e->sq, e->svr, e->sfa, e->sfc, e->skey, e->sid,
+    HIP_TRY(launch_force_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->sid,
                               e->cell_start, n,
                               e->consts, e->d_scenes, adaptive ? e->fscal : nullptr, dt, e->pm,
                               e->vel, e->acc, e->rho, e->prs, e->shape_force, st));
@@ -576,17 +576,16 @@ int sphk_create(size_t capacity, sphk_engine** out) {
     HIP_TRY(dalloc(&e->hist, ((size_t)1 << kRadixMaxBits) * ((C + kRadixTile - 1) / kRadixTile)));
     HIP_TRY(dalloc(&e->spm, C + 1));
     HIP_TRY(dalloc(&e->svr, C + 1));
-    HIP_TRY(dalloc(&e->sfa, C + 1));
-    HIP_TRY(dalloc(&e->sfc, C + 1));
+    HIP_TRY(dalloc(&e->frows, C + 1));
     HIP_TRY(dalloc(&e->sq, C + 1));
     {
         // padding slot: zero mass and weights, 1e15 away from any particle
         const float4 far_row = make_float4(1e15f, 1e15f, 1e15f, 0.0f);
         const float4 zero_row = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         HIP_TRY(hipMemcpy(e->spm + C, &far_row, sizeof(float4), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(e->sfa + C, &far_row, sizeof(float4), hipMemcpyHostToDevice));
+        const ForceRow pad_row{far_row, zero_row};
+        HIP_TRY(hipMemcpy(e->frows + C, &pad_row, sizeof(ForceRow), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(e->svr + C, &zero_row, sizeof(float4), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(e->sfc + C, &zero_row, sizeof(float4), hipMemcpyHostToDevice));
     }
     HIP_TRY(dalloc(&e->fscal, 4));
     HIP_TRY(dalloc(&e->uscal, 4));
@@ -610,7 +609,7 @@ int sphk_destroy(sphk_engine* e) {
     void* ptrs[] = {e->pm,   e->vel,      e->acc,  e->rho,      e->prs,  e->snap_pm,
                     e->snap_vel, e->snap_acc, e->keys, e->keys_alt, e->vals, e->vals_alt,
                     e->hist, e->spm,      e->svr,  e->cell_start, e->fscal, e->uscal,
-                    e->nstats, e->sfa, e->sfc, e->sq};
+                    e->nstats, e->frows, e->sq};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : {(void*)e->walk.items, (void*)e->walk.nitems, (void*)e->walk.colcnt,

@@ -96,11 +96,10 @@ struct sphk_engine {
     int shape_force = 0;
     // sorted copies, capacity + 1 rows (the last is the padding slot StepConsts::pad_slot):
     // spm (x, y, z, m), svr (vx, vy, vz, rho), and the force walk's neighbour rows written by
-    // the density walk: sfa (x, y, z, m/rho), sfc (vx, vy, vz, m P / (2 rho^2))
+    // the density walk
     float4* spm = nullptr;
     float4* svr = nullptr;
-    float4* sfa = nullptr;
-    float4* sfc = nullptr;
+    sphk::ForceRow* frows = nullptr;
     uint2* sq = nullptr;  // quantized coordinates of the candidate filter (x|y 16 bits, z 32)
     uint32_t* cell_start = nullptr;
     uint64_t cell_start_cap = 0;

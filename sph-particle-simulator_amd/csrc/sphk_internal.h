@@ -42,6 +42,12 @@ struct StepConsts {
     double qscale;      // quantized coordinate = floor(x * qscale): 256 units per cell
 };
 
+// Neighbour row of the force walk, one per sorted slot: a = (x, y, z, m/rho),
+// b = (vx, vy, vz, m P / (2 rho^2)); both halves share one 32-byte block.
+struct alignas(32) ForceRow {
+    float4 a, b;
+};
+
 // Dense cell grid of one scene (z fastest) and its place in the global cell/column
 // numbering.  Scenes are independent: no neighbour search crosses two scene grids.
 struct SceneGrid {
@@ -135,13 +141,13 @@ hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, const 
                          uint32_t* nitems, hipStream_t st);
 hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
                                float4* svr,
-                               float4* sfa, float4* sfc, const uint32_t* skey,
+                               ForceRow* rows, const uint32_t* skey,
                                const uint32_t* cell_start, uint32_t n,
                                const StepConsts& c, const SceneGrid* scenes,
                                unsigned long long* nbr_stats, int shape, hipStream_t st);
 hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
                              const float4* svr,
-                             const float4* sfa, const float4* sfc, const uint32_t* skey,
+                             const ForceRow* rows, const uint32_t* skey,
                              const uint32_t* sid,
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
                              const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,

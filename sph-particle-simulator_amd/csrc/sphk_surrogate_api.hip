@@ -198,7 +198,7 @@ int pack(sphk_engine* e, SurrogateState* s) {
 int prepare_density(sphk_engine* e) {
     if (int rc = prepare_grid(e)) return rc;
     if (int rc = build_sorted(e)) return rc;
-    HIP_TRY(launch_density_walk(e->walk, e->spm, e->sq, e->svr, e->sfa, e->sfc, e->skey, e->cell_start,
+    HIP_TRY(launch_density_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->cell_start,
                                 (uint32_t)e->n, e->consts, e->d_scenes, nullptr,
                                 e->shape_density, e->stream));
     return SPHK_OK;

@@ -105,8 +105,7 @@ template <class Shape>
 __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const uint2* __restrict__ sq, float4* __restrict__ svr,
-    float4* __restrict__ sfa,
-    float4* __restrict__ sfc, const uint32_t* __restrict__ skey,
+    ForceRow* __restrict__ rows, const uint32_t* __restrict__ skey,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     uint32_t* __restrict__ gmask, uint32_t* __restrict__ gwj0, uint32_t* __restrict__ gnw,
     uint32_t* __restrict__ ggrp, unsigned long long* __restrict__ stats) {
@@ -159,8 +158,8 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
         // m_j (P_i + P_j) / (2 rho_j) = (P_i / 2) B_j + C_j
         const float B = pi.w / rho;
         const float P = c.k_gas * (rho - c.rho0);
-        sfa[s] = make_float4(pi.x, pi.y, pi.z, B);
-        sfc[s] = make_float4(v.x, v.y, v.z, 0.5f * B * P);
+        rows[s].a = make_float4(pi.x, pi.y, pi.z, B);
+        rows[s].b = make_float4(v.x, v.y, v.z, 0.5f * B * P);
     }
     if (stats) add_nbr_stats(cnt, stats, lane);
 }
@@ -224,7 +223,7 @@ __device__ __forceinline__ float3 final_accel(float3 fp, float3 fv, float m_i, f
     return make_float3(fx * inv_m, (fy + c.grav * m) * inv_m, fz * inv_m);
 }
 
-// Pressure and viscosity sums of one target over neighbour rows (a = sfa[j], b = sfc[j]).
+// Pressure and viscosity sums of one target over neighbour rows (ForceRow a, b).
 // Pairs the filter let through beyond the radius (q >= 2), the self pair (r = 0, v_j = v_i)
 // and the padding slot (B = C = 0) all add exactly zero.
 struct ForceSums {
@@ -259,21 +258,20 @@ template <int kSeg>
 __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                            const uint32_t* __restrict__ lmask,
                                            const uint32_t* __restrict__ lwj0, uint32_t lane,
-                                           const float4* __restrict__ sfa,
-                                           const float4* __restrict__ sfc, const float4& pi,
+                                           const ForceRow* __restrict__ rows, const float4& pi,
                                            const float3& vi, float p_half, const StepConsts& c,
                                            ForceSums& f) {
     if (steps == 0) return;
     uint32_t ja = bw.next(lmask, lwj0, lane);
-    float4 a0 = sfa[ja], b0 = sfc[ja];
+    float4 a0 = rows[ja].a, b0 = rows[ja].b;
     uint32_t t = 0;
     for (; t + 2 <= steps; t += 2) {
         const uint32_t jb = bw.next(lmask, lwj0, lane);
-        const float4 a1 = sfa[jb], b1 = sfc[jb];
+        const float4 a1 = rows[jb].a, b1 = rows[jb].b;
         f.add(pi, vi, p_half, a0, b0, c);
         ja = bw.next(lmask, lwj0, lane);
-        a0 = sfa[ja];
-        b0 = sfc[ja];
+        a0 = rows[ja].a;
+        b0 = rows[ja].b;
         f.add(pi, vi, p_half, a1, b1, c);
     }
     if (t < steps) f.add(pi, vi, p_half, a0, b0, c);
@@ -283,8 +281,7 @@ template <class Shape>
 __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const uint2* __restrict__ sq,
-    const float4* __restrict__ svr, const float4* __restrict__ sfa,
-    const float4* __restrict__ sfc,
+    const float4* __restrict__ svr, const ForceRow* __restrict__ rows,
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sid,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     const uint32_t* __restrict__ gmask, const uint32_t* __restrict__ gwj0,
@@ -343,7 +340,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
         wave_sync();
         BitWalk<kSeg> bw;
         bw.ne = sl.nonempty;
-        force_bits<kSeg>(bw, wave_max(sl.count), lmask, lwj0, lane, sfa, sfc, pi, vi, p_half, c,
+        force_bits<kSeg>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi, p_half, c,
                          f);
         seg_base += nw;
         wave_sync();
@@ -488,7 +485,7 @@ static uint32_t grid_for(uint32_t chunks) {
 
 hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
                                float4* svr,
-                               float4* sfa, float4* sfc, const uint32_t* skey,
+                               ForceRow* rows, const uint32_t* skey,
                                const uint32_t* cell_start, uint32_t n,
                                const StepConsts& c, const SceneGrid* scenes,
                                unsigned long long* nbr_stats, int shape, hipStream_t st) {
@@ -496,7 +493,7 @@ hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const u
     const uint32_t chunks = max_chunks(n, c);
     SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_density_walk<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, sq, svr, sfa, sfc, skey, cell_start, c,
+                                                  spm, sq, svr, rows, skey, cell_start, c,
                                                   scenes, wb.mask, wb.wj0, wb.nwords,
                                                   wb.groups, nbr_stats));
     return hipGetLastError();
@@ -504,7 +501,7 @@ hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const u
 
 hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
                              const float4* svr,
-                             const float4* sfa, const float4* sfc, const uint32_t* skey,
+                             const ForceRow* rows, const uint32_t* skey,
                              const uint32_t* sid,
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
                              const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
@@ -514,7 +511,7 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
     const uint32_t chunks = max_chunks(n, c);
     SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_force_walk<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, sq, svr, sfa, sfc, skey, sid,
+                                                  spm, sq, svr, rows, skey, sid,
                                                   cell_start, c,
                                                   scenes,
                                                   wb.mask, wb.wj0, wb.nwords, wb.groups, dt_dev,
