@@ -32,9 +32,12 @@ ALG_BYTES = {"density": 16.0, "force": 52.0, "step": 149.0}
 # HBM bytes the force walk moves per particle on the 1M dam break, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE scaled by a 256 MiB device copy (fetch x2.0, write x1.0); see
 # profiles/force_walk_dam_break_1M_traffic.md.  Mostly the stored neighbour words it reads back.
-FORCE_TRAFFIC_BYTES_PER_PARTICLE = 499.0
+FORCE_TRAFFIC_BYTES_PER_PARTICLE = 539.0
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3  # published; counts packed FP32 (two lanes per VALU slot)
+FP32_PEAK_UNPACKED_TFLOPS = 78.6  # one wave64 FP32 instruction per SIMD per 4 cycles
+# algorithmic FP32 work per neighbour pair (FMA = 2, sqrt/rsqrt = 1): density 17, force 37
+FLOP_PER_PAIR = 17 + 37
 BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
 
 
@@ -309,9 +312,8 @@ def main():
     achieved = ALG_BYTES["force"] * n / t_force / 1e9
     mean_nbrs = st.total_neighbors / max(st.total_neighbor_queries, 1)
     ups = n * args.steps / elapsed
-    # FP32 work of the two walks (27 cells x ~64 candidates x 8 flops x 2 passes, plus kernel
-    # evaluation per neighbour: ~10 flops density, ~40 force)
-    flops_per_update = 2 * 27 * 64 * 8 + mean_nbrs * (10 + 40) + 20
+    # FP32 pair arithmetic of the two walks (the candidate filter is integer work)
+    flops_per_update = FLOP_PER_PAIR * mean_nbrs
     out = {
         "metric": METRIC,
         "value": ups,
@@ -346,10 +348,12 @@ def main():
             "kernel_ms": t_force * 1e3,
         },
         "valu": {
-            "flops_per_update": flops_per_update,
+            "fp32_flop_per_update": flops_per_update,
             "achieved_tflops": flops_per_update * ups / 1e12,
             "peak_tflops": FP32_PEAK_TFLOPS,
             "frac": flops_per_update * ups / 1e12 / FP32_PEAK_TFLOPS,
+            "peak_unpacked_tflops": FP32_PEAK_UNPACKED_TFLOPS,
+            "frac_unpacked": flops_per_update * ups / 1e12 / FP32_PEAK_UNPACKED_TFLOPS,
         },
         "phase_ms": {"neighbor_search": t_nbr * 1e3, "density": t_dens * 1e3,
                      "force_integrate": t_force * 1e3, "step_events": st.total_time / k * 1e3},

@@ -175,8 +175,8 @@ int grid_for_ext(const SceneHost& sc, float radius, Grid* g) {
     return SPHK_OK;
 }
 
-__global__ void k_bbox(const float4* __restrict__ pm, uint32_t begin, uint32_t end,
-                       uint32_t* __restrict__ out7) {
+__global__ __launch_bounds__(256) void k_bbox(const float4* __restrict__ pm, uint32_t begin,
+                                              uint32_t end, uint32_t* __restrict__ out7) {
     // out7 = {min x,y,z, max x,y,z as order-preserving uints, any}
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -211,13 +211,35 @@ __global__ void k_bbox(const float4* __restrict__ pm, uint32_t begin, uint32_t e
             ohi[d] = b > ohi[d] ? b : ohi[d];
         }
     }
+    // then across the workgroup's wavefronts in LDS: one set of atomics per workgroup (all
+    // workgroups update the same seven words, so their number bounds the kernel's time)
+    __shared__ unsigned red[4][7];
+    const unsigned wave = threadIdx.x >> 6;
     const bool any = __ballot(found) != 0ull;
-    if (any && (threadIdx.x & 63u) == 0u) {
+    if ((threadIdx.x & 63u) == 0u) {
         for (int d = 0; d < 3; ++d) {
-            atomicMin(&out7[d], olo[d]);
-            atomicMax(&out7[3 + d], ohi[d]);
+            red[wave][d] = olo[d];
+            red[wave][3 + d] = ohi[d];
         }
-        atomicOr(&out7[6], 1u);
+        red[wave][6] = any ? 1u : 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned nw = blockDim.x >> 6;
+        for (unsigned w = 1; w < nw; ++w) {
+            for (int d = 0; d < 3; ++d) {
+                red[0][d] = red[w][d] < red[0][d] ? red[w][d] : red[0][d];
+                red[0][3 + d] = red[w][3 + d] > red[0][3 + d] ? red[w][3 + d] : red[0][3 + d];
+            }
+            red[0][6] |= red[w][6];
+        }
+        if (red[0][6]) {
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(&out7[d], red[0][d]);
+                atomicMax(&out7[3 + d], red[0][3 + d]);
+            }
+            atomicOr(&out7[6], 1u);
+        }
     }
 }
 
@@ -241,7 +263,7 @@ int device_bbox(sphk_engine* e) {
     for (size_t k = 0; k < S; ++k) {
         const uint64_t b = e->scenes[k].begin, en = std::min<uint64_t>(e->scenes[k].end, e->n);
         if (en > b) {
-            uint32_t blocks = (uint32_t)std::min<uint64_t>((en - b + 255) / 256, 1024);
+            uint32_t blocks = (uint32_t)std::min<uint64_t>((en - b + 255) / 256, 256);
             hipLaunchKernelGGL(k_bbox, dim3(blocks), dim3(256), 0, e->stream, e->pm, (uint32_t)b,
                                (uint32_t)en, buf + 8 * k);
             HIP_TRY(hipGetLastError());
