@@ -4,7 +4,8 @@
 //   cell_keys      dense cell index of every particle, caller order
 //   radix sort     stable LSD sort of (cell, caller id) -> cell-sorted order
 //   cell_start     occupancy prefix of the dense grid (binary search per cell)
-//   gather         cell-sorted float4 copies of (x,y,z,m) and (vx,vy,vz,-)
+//   gather         cell-sorted float4 copies of (x,y,z,m) and (vx,vy,vz,-), quantized
+//                  coordinates of the candidate filter
 //   chunks         64-particle work items per (x,y) column of the grid
 //   density, force walks   (sphk_walk.hip)
 #include <hip/hip_runtime.h>
@@ -367,7 +368,15 @@ __global__ __launch_bounds__(256) void k_cfl_reduce(const float4* __restrict__ v
         float o = __shfl_xor(m, off);
         if (m < o) m = o;
     }
-    if (lane_id() == 0 && m > 0.0f) atomicMax(scratch, __float_as_uint(m));
+    // one atomic per workgroup: every workgroup updates the same word
+    __shared__ float red[4];
+    if (lane_id() == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < (blockDim.x >> 6); ++w)
+            if (m < red[w]) m = red[w];
+        if (m > 0.0f) atomicMax(scratch, __float_as_uint(m));
+    }
 }
 
 __global__ void k_cfl_finalize(const float4* __restrict__ acc, uint32_t n, float cfl, float h,
@@ -387,7 +396,7 @@ __global__ void k_cfl_finalize(const float4* __restrict__ acc, uint32_t n, float
 hipError_t launch_cfl(const float4* vel, const float4* acc, uint32_t n, const sphk_params& p,
                       uint32_t* scratch_u32, float* dt_dev, hipStream_t st) {
     uint32_t blocks = (n + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
+    if (blocks > 256) blocks = 256;
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_cfl_reduce, dim3(blocks), dim3(256), 0, st, vel, n, scratch_u32);
     hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(1), 0, st, acc, n, p.CFL_factor,
