@@ -252,8 +252,9 @@ struct ForceSums {
     }
 };
 
-// The wavefront runs its largest lane count; rows are requested one neighbour ahead, in two
-// alternating register sets.
+// The wavefront runs its largest lane count; rows are requested two neighbours ahead, in
+// three rotating register sets (a gather from L2 takes several hundred cycles, one step of
+// pair arithmetic about a hundred).
 template <int kSeg>
 __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                            const uint32_t* __restrict__ lmask,
@@ -262,19 +263,26 @@ __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                            const float3& vi, float p_half, const StepConsts& c,
                                            ForceSums& f) {
     if (steps == 0) return;
-    uint32_t ja = bw.next(lmask, lwj0, lane);
-    float4 a0 = rows[ja].a, b0 = rows[ja].b;
+    uint32_t j = bw.next(lmask, lwj0, lane);
+    float4 a0 = rows[j].a, b0 = rows[j].b;
+    j = bw.next(lmask, lwj0, lane);
+    float4 a1 = rows[j].a, b1 = rows[j].b;
     uint32_t t = 0;
-    for (; t + 2 <= steps; t += 2) {
-        const uint32_t jb = bw.next(lmask, lwj0, lane);
-        const float4 a1 = rows[jb].a, b1 = rows[jb].b;
+    for (; t + 3 <= steps; t += 3) {
+        j = bw.next(lmask, lwj0, lane);
+        const float4 a2 = rows[j].a, b2 = rows[j].b;
         f.add(pi, vi, p_half, a0, b0, c);
-        ja = bw.next(lmask, lwj0, lane);
-        a0 = rows[ja].a;
-        b0 = rows[ja].b;
+        j = bw.next(lmask, lwj0, lane);
+        a0 = rows[j].a;
+        b0 = rows[j].b;
         f.add(pi, vi, p_half, a1, b1, c);
+        j = bw.next(lmask, lwj0, lane);
+        a1 = rows[j].a;
+        b1 = rows[j].b;
+        f.add(pi, vi, p_half, a2, b2, c);
     }
     if (t < steps) f.add(pi, vi, p_half, a0, b0, c);
+    if (t + 1 < steps) f.add(pi, vi, p_half, a1, b1, c);
 }
 
 template <class Shape>
