@@ -84,20 +84,36 @@ uint32_t max_chunks(uint32_t n, const StepConsts& c) { return (n + 63u) / 64u + 
 // runs over the set bits, self included.  Stores the neighbour words of every chunk and the
 // force walk's neighbour rows of every particle.
 // ---------------------------------------------------------------------------
+template <bool kCount>
+__device__ __forceinline__ float density_pair(const float4& pi, const float4& q,
+                                              const StepConsts& c, float acc, uint32_t& cnt) {
+    const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
+    if (kCount) cnt += d2 <= c.r2 ? 1u : 0u;  // the reference's test
+    // pairs the filter let through beyond the radius have q >= 2 and weigh zero
+    return acc + q.w * w_shape(__builtin_amdgcn_sqrtf(d2) * c.inv_h);
+}
+
+// Neighbour positions are requested two steps ahead, in three rotating register sets.
 template <bool kCount, int kSeg>
 __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                               const uint32_t* __restrict__ lmask,
                                               const uint32_t* __restrict__ lwj0, uint32_t lane,
                                               const float4* __restrict__ spm, const float4& pi,
                                               const StepConsts& c, float acc, uint32_t& cnt) {
-    for (uint32_t t = 0; t < steps; ++t) {
-        const uint32_t j = bw.next(lmask, lwj0, lane);
-        const float4 q = spm[j];
-        const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
-        if (kCount) cnt += d2 <= c.r2 ? 1u : 0u;  // the reference's test
-        // pairs the filter let through beyond the radius have q >= 2 and weigh zero
-        acc += q.w * w_shape(__builtin_amdgcn_sqrtf(d2) * c.inv_h);
+    if (steps == 0) return acc;
+    float4 q0 = spm[bw.next(lmask, lwj0, lane)];
+    float4 q1 = spm[bw.next(lmask, lwj0, lane)];
+    uint32_t t = 0;
+    for (; t + 3 <= steps; t += 3) {
+        const float4 q2 = spm[bw.next(lmask, lwj0, lane)];
+        acc = density_pair<kCount>(pi, q0, c, acc, cnt);
+        q0 = spm[bw.next(lmask, lwj0, lane)];
+        acc = density_pair<kCount>(pi, q1, c, acc, cnt);
+        q1 = spm[bw.next(lmask, lwj0, lane)];
+        acc = density_pair<kCount>(pi, q2, c, acc, cnt);
     }
+    if (t < steps) acc = density_pair<kCount>(pi, q0, c, acc, cnt);
+    if (t + 1 < steps) acc = density_pair<kCount>(pi, q1, c, acc, cnt);
     return acc;
 }
 
