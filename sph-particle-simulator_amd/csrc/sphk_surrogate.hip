@@ -51,10 +51,9 @@ __device__ __forceinline__ float w_shape_ieee(float q) {
     return q <= 1.0f ? win : (q <= 2.0f ? wout : 0.0f);
 }
 
-// Edge features of one pair in FP32, zero when `edge` is false.
-__device__ __forceinline__ void edge_values(bool edge, float4 pi, float4 vi, float4 pj,
-                                            float4 vj, float d2, const SurScales& sc,
-                                            float (&f)[9]) {
+// Edge features of one pair in FP32 (any values for a non-edge: the caller zeroes them).
+__device__ __forceinline__ void edge_values(float4 pi, float4 vi, float4 pj, float4 vj,
+                                            float d2, const SurScales& sc, float (&f)[9]) {
 #pragma clang fp contract(off)
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float rl = __builtin_amdgcn_sqrtf(d2);
@@ -68,20 +67,29 @@ __device__ __forceinline__ void edge_values(bool edge, float4 pi, float4 vi, flo
     f[6] = rl * sc.inv_r;
     f[7] = w * sc.w_norm;
     f[8] = (vj.w - sc.rho0) * sc.rho_scale;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) f[k] = edge ? f[k] : 0.0f;
 }
 
-// Edge features of one pair rounded to bf16: features 0..7 as one operand vector, feature 8
-// (neighbour density) apart.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Edge features of one pair rounded to bf16 (round to nearest even, two per conversion),
+// zero when `edge` is false: features 0..7 as one operand vector, feature 8 (neighbour
+// density) apart.
 __device__ __forceinline__ void edge_features(bool edge, float4 pi, float4 vi, float4 pj,
                                               float4 vj, float d2, const SurScales& sc,
                                               bf16x8& e, __bf16& e8) {
     float f[9];
-    edge_values(edge, pi, vi, pj, vj, d2, sc, f);
+    edge_values(pi, vi, pj, vj, d2, sc, f);
+    uint4 w;
+    uint32_t* wp = &w.x;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) e[k] = (__bf16)f[k];
-    e8 = (__bf16)f[8];
+    for (int k = 0; k < 4; ++k) {
+        const f32x2 v = {f[2 * k], f[2 * k + 1]};
+        const uint32_t bits = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+        wp[k] = edge ? bits : 0u;
+    }
+    e = __builtin_bit_cast(bf16x8, w);
+    e8 = edge ? (__bf16)f[8] : (__bf16)0.0f;
 }
 
 __device__ __forceinline__ uint32_t bits16(__bf16 x) {
