@@ -7,16 +7,19 @@
 // to +-2 on an axis when `axis_edge_flags` proves a pair two cells apart possible for some
 // lane of the chunk).  Each column contributes one contiguous range of sorted slots.
 //
-// Phase 1 (test): candidates are walked in the reference's (dx, dy, dz) order, one at a time
-//   for the whole wave, so the candidate index is wave-uniform and its (x, y, z, m) comes
-//   through the scalar cache.  Each lane evaluates the exact FP32 test and sets one bit of
-//   a 32-bit word per 32 candidates.  Words never straddle two columns; `wj0[w]` is the
-//   sorted slot of the first candidate of word w.
-// Phase 2 (evaluate): every lane walks only its own set bits, in word order (= reference
-//   neighbour order), and evaluates the kernel for those pairs.
-// The density pass stores the words of each chunk in HBM (masks, wj0, word count); the force
-// pass reads them back instead of testing again.  Chunks with more than kMaxWords words
-// (pile-ups of thousands of particles per cell) are re-tested by the force pass.
+// Phase 1 (test): candidates are walked one at a time for the whole wave, column by column
+//   (corner columns, edge columns, centre: see kColumnOrder), so the candidate index is
+//   wave-uniform and its quantized coordinates come through the scalar cache.  Each lane
+//   applies the conservative integer filter (push_quant) and sets one bit of a 32-bit word per
+//   32 candidates.  Words never straddle two columns; `wj0[w]` is the sorted slot of the
+//   first candidate of word w.  Segments of up to 31 words hold whole column groups.
+// Phase 2 (evaluate): every lane walks only its own set bits and evaluates the kernel for
+//   those pairs (pairs the filter let through beyond the radius weigh exactly zero).
+// The density pass stores the words of each chunk in HBM (masks, wj0, word count, group
+// starts) and the force rows of each particle; the force pass reads them back instead of
+// testing again.  Chunks with more than kMaxWords words (pile-ups of thousands of particles
+// per cell) are re-tested by the force pass.  The neighbour-list export walks the reference's
+// (dx, dy, dz, id) order.
 #include <hip/hip_runtime.h>
 
 #include <climits>
