@@ -196,6 +196,30 @@ def test_edge_pairs_in_all_directions_against_brute_force(sphk, oracle):
         assert k + 1 in idx[offs[k]:offs[k + 1]].tolist()
 
 
+def test_far_from_origin_against_brute_force(sphk, oracle):
+    # Coordinates near 1000 (cell index ~25,000): the quantized filter keeps its coordinates
+    # modulo 2^16 (x, y) and 2^32 (z), so neighbour differences must survive the wrap-around.
+    rng = np.random.default_rng(41)
+    r2 = np.float32(R2)
+    base = np.array([1000.3, -777.7, 513.1])
+    pts = [(base + rng.uniform(0, 0.25, 3)).astype(np.float32) for _ in range(2500)]
+    for _ in range(200):
+        b = (base + rng.uniform(0, 0.25, 3)).astype(np.float32)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        pts += [b, _boundary_partner(b, u, r2)]
+    pos = np.array(pts, np.float32)
+    lo, hi = pos.min(0) - 0.5, pos.max(0) + 0.5
+    bounds = dict(zip(("xmin", "ymin", "zmin"), map(float, lo)))
+    bounds.update(zip(("xmax", "ymax", "zmax"), map(float, hi)))
+    e = sphk.Engine(len(pos), sphk.default_params(**bounds))
+    e.upload(pos)
+    offs, idx = e.neighbors()
+    boffs, bidx = oracle.brute_neighbors(pos, r2)
+    for i in range(len(pos)):
+        assert sorted(idx[offs[i]:offs[i + 1]].tolist()) == bidx[boffs[i]:boffs[i + 1]].tolist()
+
+
 def test_nonfinite_particles(sphk, oracle):
     pos = np.array([[0.1, 0.1, 0.1], [np.nan, 0.1, 0.1], [0.11, 0.1, 0.1], [0.1, np.inf, 0.1],
                     [0.12, 0.1, 0.1]], np.float32)
