@@ -1002,12 +1002,8 @@ int sphk_snapshot_save(sphk_engine* e) {
     if (int rc = check_engine(e)) return rc;
     if (int rc = set_device(e)) return rc;
     const size_t n = e->n;
-    if (n) {
-        HIP_TRY(hipMemcpyAsync(e->snap_pm, e->pm, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
-        HIP_TRY(hipMemcpyAsync(e->snap_vel, e->vel, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
-        HIP_TRY(hipMemcpyAsync(e->snap_acc, e->acc, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
-    }
-    HIP_TRY(hipMemcpyAsync(e->fscal + 2, e->fscal + 1, sizeof(float), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(launch_copy_state(e->pm, e->vel, e->acc, e->snap_pm, e->snap_vel, e->snap_acc, n,
+                              e->fscal + 1, e->fscal + 2, e->stream));
     e->snap_n = n;
     e->snap_steps = e->step_count;
     e->snap_scenes = e->scenes;
@@ -1021,12 +1017,8 @@ int sphk_snapshot_restore(sphk_engine* e) {
     if (!e->has_snapshot) return set_error(SPHK_E_INVALID, "no snapshot saved");
     if (int rc = set_device(e)) return rc;
     const size_t n = e->snap_n;
-    if (n) {
-        HIP_TRY(hipMemcpyAsync(e->pm, e->snap_pm, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
-        HIP_TRY(hipMemcpyAsync(e->vel, e->snap_vel, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
-        HIP_TRY(hipMemcpyAsync(e->acc, e->snap_acc, n * sizeof(float4), hipMemcpyDeviceToDevice, e->stream));
-    }
-    HIP_TRY(hipMemcpyAsync(e->fscal + 1, e->fscal + 2, sizeof(float), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(launch_copy_state(e->snap_pm, e->snap_vel, e->snap_acc, e->pm, e->vel, e->acc, n,
+                              e->fscal + 2, e->fscal + 1, e->stream));
     e->n = n;
     e->step_count = e->snap_steps;
     e->sorted_valid = false;

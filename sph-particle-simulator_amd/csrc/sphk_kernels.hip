@@ -10,6 +10,7 @@
 //   density, force walks   (sphk_walk.hip)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 
@@ -401,6 +402,35 @@ hipError_t launch_cfl(const float4* vel, const float4* acc, uint32_t n, const sp
     hipLaunchKernelGGL(k_cfl_reduce, dim3(blocks), dim3(256), 0, st, vel, n, scratch_u32);
     hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(1), 0, st, acc, n, p.CFL_factor,
                        p.smoothing_length, p.timestep, scratch_u32, dt_dev);
+    return hipGetLastError();
+}
+
+// Copies three float4 arrays of n rows and one scalar in one launch (state snapshot and
+// restore: positions, velocities, accelerations and the clock).
+__global__ __launch_bounds__(256) void k_copy_state(const float4* __restrict__ a0,
+                                                    const float4* __restrict__ a1,
+                                                    const float4* __restrict__ a2,
+                                                    float4* __restrict__ b0,
+                                                    float4* __restrict__ b1,
+                                                    float4* __restrict__ b2, size_t n,
+                                                    const float* __restrict__ t_src,
+                                                    float* __restrict__ t_dst) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4 x0 = a0[i], x1 = a1[i], x2 = a2[i];
+        b0[i] = x0;
+        b1[i] = x1;
+        b2[i] = x2;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *t_dst = *t_src;
+}
+
+hipError_t launch_copy_state(const float4* a0, const float4* a1, const float4* a2, float4* b0,
+                             float4* b1, float4* b2, size_t n, const float* t_src,
+                             float* t_dst, hipStream_t st) {
+    const size_t blocks = std::min<size_t>(std::max<size_t>((n + 255) / 256, 1), 4096);
+    hipLaunchKernelGGL(k_copy_state, dim3((uint32_t)blocks), dim3(256), 0, st, a0, a1, a2, b0,
+                       b1, b2, n, t_src, t_dst);
     return hipGetLastError();
 }
 
