@@ -171,7 +171,6 @@ __device__ __forceinline__ int column_group(int k) { return k < 4 ? 0 : k < 8 ? 
 struct Cursor {
     int k;            // position in the column order
     int order;        // 0 balanced, 1 reference
-    int dx, dy;       // current column
     int group;        // balance group of the current column (0 in the reference order)
     bool fresh;       // the current column opens its group
     uint32_t j, end;  // current slot and end of the current column range
@@ -210,8 +209,6 @@ __device__ __forceinline__ void cursor_seek(Cursor& cu, const Chunk& ch,
         const int g = cu.order == 0 ? column_group(k) : 0;
         cu.fresh = k == 0 || g != cu.group;
         cu.k = k;
-        cu.dx = dx;
-        cu.dy = dy;
         cu.group = g;
         column_range(ch, dx, dy, cs, c, cu.j, cu.end);
         return;
