@@ -24,7 +24,7 @@ namespace sphk {
 struct StepConsts {
     float inv_cell;  // 1.0f / neighbor_search_radius
     float r2;        // neighbor_search_radius^2
-    float r2_filter; // r2 raised by 8 ulp: threshold of the contracted candidate filter
+    float r2_filter; // r2 raised by 8 ulp: threshold of the contracted FP32 candidate filter
     float h, inv_h;
     float sigma;     // 1/(pi h^3) (x 3/2 for the consistent formulation)
     float w0;        // W(0) = sigma * (2/3)

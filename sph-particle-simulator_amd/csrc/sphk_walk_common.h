@@ -46,8 +46,9 @@ __device__ __forceinline__ float dist2_exact(float ax, float ay, float az, float
     return (tx + ty) + tz;
 }
 
-// Contracted d^2 for the candidate filter: fl(dx^2 + fl(dy^2 + fl(dz^2))) with two fused
-// multiply-adds.  Both roundings stay within 3 units of 2^-24 of the true sum, so the two
+// Contracted d^2 for the FP32 candidate filter of the surrogate's in-kernel list building
+// (the SPH walks use the integer filter, push_quant): fl(dx^2 + fl(dy^2 + fl(dz^2))) with
+// two fused multiply-adds.  Both roundings stay within 3 units of 2^-24 of the true sum, so the two
 // values differ by under 7 units of 2^-24 relative, and
 // testing it against r2 raised by 8 ulp (StepConsts::r2_filter) keeps every pair the exact
 // test keeps; the few extra pairs are dropped by the exact test where the pair is used.
