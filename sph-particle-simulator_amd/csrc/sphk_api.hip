@@ -370,6 +370,19 @@ int ensure_grid(sphk_engine* e) {
         wb.chunk_cap = want;
         ++e->buffer_gen;
     }
+    if (!e->shapes_explicit)
+        e->shape_density = e->shape_force = e->n < kSplitBelow ? kShapeSmall : kShapeLarge;
+    // per-part sums of the split walk shapes: three sums per slot, only while selected
+    if ((walk_shape_split(e->shape_density) || walk_shape_split(e->shape_force)) &&
+        wb.part_cap < e->cap) {
+        for (void* p : {(void*)wb.part_acc, (void*)wb.part_cnt, (void*)wb.part_force})
+            if (p) HIP_TRY(hipFree(p));
+        HIP_TRY(dalloc(&wb.part_acc, kParts * e->cap));
+        HIP_TRY(dalloc(&wb.part_cnt, kParts * e->cap));
+        HIP_TRY(dalloc(&wb.part_force, 2 * kParts * e->cap));
+        wb.part_cap = e->cap;
+        ++e->buffer_gen;
+    }
     uint64_t ccap = wb.col_cap;
     if (int rc = grow(&wb.colcnt, ccap, ncols + 1, e->buffer_gen)) return rc;
     wb.col_cap = ccap;
@@ -636,7 +649,8 @@ int sphk_destroy(sphk_engine* e) {
         if (p) (void)hipFree(p);
     for (void* p : {(void*)e->walk.items, (void*)e->walk.nitems, (void*)e->walk.colcnt,
                     (void*)e->walk.mask, (void*)e->walk.wj0, (void*)e->walk.nwords,
-                    (void*)e->walk.groups, (void*)e->scan_scratch, (void*)e->d_scenes})
+                    (void*)e->walk.groups, (void*)e->walk.part_acc, (void*)e->walk.part_cnt,
+                    (void*)e->walk.part_force, (void*)e->scan_scratch, (void*)e->d_scenes})
         if (p) (void)hipFree(p);
     if (e->sur) surrogate_free(e->sur);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
@@ -1196,6 +1210,11 @@ int sphk_set_walk_shapes(sphk_engine* e, int density_shape, int force_shape) {
         return set_error(SPHK_E_INVALID, "walk shapes must be in [0, %d)", kNumWalkShapes);
     e->shape_density = density_shape;
     e->shape_force = force_shape;
+    e->shapes_explicit = true;
+    // a split shape needs the per-part sums, sized with the grid buffers
+    if ((walk_shape_split(density_shape) || walk_shape_split(force_shape)) &&
+        e->walk.part_cap < e->cap)
+        e->grid_valid = false;
     return SPHK_OK;
 }
 

@@ -94,6 +94,7 @@ struct sphk_engine {
     WalkBuffers walk;
     int shape_density = 0;  // walk launch shapes (walk_shape_name), set by sphk_set_walk_shapes
     int shape_force = 0;
+    bool shapes_explicit = false;  // false: ensure_grid picks the shapes by particle count
     // sorted copies, capacity + 1 rows (the last is the padding slot StepConsts::pad_slot):
     // spm (x, y, z, m), svr (vx, vy, vz, rho), and the force walk's neighbour rows written by
     // the density walk

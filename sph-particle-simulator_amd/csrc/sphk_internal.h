@@ -74,8 +74,19 @@ constexpr int kRadixMaxBits = 11;
 
 // neighbour walks: words of 32 candidates kept in LDS per wave, and per chunk in HBM
 constexpr int kMaxWords = 128;
-constexpr int kNumWalkShapes = 5;  // launch shapes selectable per walk (see walk_shape_name)
+constexpr int kNumWalkShapes = 7;  // launch shapes selectable per walk (see walk_shape_name)
 const char* walk_shape_name(int v);
+// Parts of a chunk's candidate columns (corners, edges, centre with the outer ring).
+constexpr int kParts = 3;
+// Shapes that walk each chunk with three wavefronts, one per part of its candidate columns;
+// they need the per-part sums of WalkBuffers.
+constexpr bool walk_shape_split(int v) { return v >= 5; }
+// Shapes an engine uses until the caller sets its own: below kSplitBelow particles the
+// single-wavefront walk leaves most of the GPU idle and the split seg32_waves4 is faster;
+// above, seg32_waves4 (dam-break blocks of 10K to 1M particles on MI355X).
+constexpr int kShapeSmall = 6;
+constexpr int kShapeLarge = 1;
+constexpr uint64_t kSplitBelow = 65536;
 
 // HBM buffers of the chunked walks (sized for max_chunks())
 struct WalkBuffers {
@@ -88,6 +99,20 @@ struct WalkBuffers {
     uint32_t* groups = nullptr;   // [chunks] word index of column groups 1-3, 8 bits each
     size_t chunk_cap = 0;
     size_t col_cap = 0;
+    // per-part sums of the split shapes, [3][part_cap] by sorted slot (allocated only
+    // while a split shape is selected)
+    float* part_acc = nullptr;     // density sums
+    uint32_t* part_cnt = nullptr;  // neighbour counts
+    float4* part_force = nullptr;  // [3][part_cap][2]: pressure sum, viscosity sum
+    size_t part_cap = 0;
+};
+
+// Device view of the per-part sums.
+struct PartSums {
+    float* acc;
+    uint32_t* cnt;
+    float4* force;
+    size_t cap;
 };
 
 // Scene lookups by binary search over the scene table (few scenes; uniform per wave where

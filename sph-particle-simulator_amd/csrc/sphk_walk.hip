@@ -120,14 +120,31 @@ __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
     return acc;
 }
 
-template <class Shape>
+// Density, the force rows of the particle (B = m/rho and C = B P / 2, so that
+// m_j (P_i + P_j) / (2 rho_j) = (P_i / 2) B_j + C_j) and rho in svr.w.
+__device__ __forceinline__ void density_epilogue(float sum, const float4& pi, uint32_t s,
+                                                 const StepConsts& c, float4* __restrict__ svr,
+                                                 ForceRow* __restrict__ rows) {
+    const float rho = c.sigma * sum;
+    float4 v = svr[s];
+    v.w = rho;
+    svr[s] = v;
+    const float B = pi.w / rho;
+    const float P = c.k_gas * (rho - c.rho0);
+    rows[s].a = make_float4(pi.x, pi.y, pi.z, B);
+    rows[s].b = make_float4(v.x, v.y, v.z, 0.5f * B * P);
+}
+
+// Density pass over one chunk (kSplit false: all parts; true: the part of this wavefront,
+// three wavefronts per chunk, the sums finished by k_density_finish).
+template <class Shape, bool kSplit>
 __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const uint2* __restrict__ sq, float4* __restrict__ svr,
     ForceRow* __restrict__ rows, const uint32_t* __restrict__ skey,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     uint32_t* __restrict__ gmask, uint32_t* __restrict__ gwj0, uint32_t* __restrict__ gnw,
-    uint32_t* __restrict__ ggrp, unsigned long long* __restrict__ stats) {
+    uint32_t* __restrict__ ggrp, PartSums ps, unsigned long long* __restrict__ stats) {
     constexpr int kSeg = Shape::seg;
     __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
     __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
@@ -135,7 +152,9 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t* lmask = lmask_all[wave];
     uint32_t* lwj0 = lwj0_all[wave];
-    const uint32_t item = swizzled_block() * Shape::waves + wave;
+    const uint32_t wid = swizzled_block() * Shape::waves + wave;
+    const uint32_t item = kSplit ? wid / kParts : wid;
+    const int part = kSplit ? (int)(wid % kParts) : -1;
     Chunk ch;
     uint32_t s;
     bool active;
@@ -144,41 +163,71 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     const uint2 qi = active ? sq[s] : make_uint2(0u, 0u);
     init_sentinel<kSeg>(lmask, lwj0, lane, c);
     Cursor cu;
-    cursor_begin(cu, ch, cs, c);
-    if (lane == 0) ggrp[item] = pack_groups(cu);
-    float acc = 0.0f;
+    cursor_begin(cu, ch, cs, c, 0, part);
+    if (lane == 0 && part <= 0) {
+        ggrp[item] = pack_groups(cu);
+        gnw[item] = (uint32_t)(cu.gw0 + cu.gw1 + cu.gw2 + cu.gw3);
+    }
+    // per-part running sum `cur`, parts folded into `total` as (p0 + p1) + p2
+    float total = 0.0f, cur = 0.0f;
+    int cur_part = 0;
     uint32_t cnt = 0;
-    int seg_base = 0;
+    int seg_base = kSplit ? part_word_base(cu, part) : 0;
     for (;;) {
         SegLane sl;
-        const int nw =
-            phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane, sl);
+        int seg_part;
+        const int nw = phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane,
+                                            sl, seg_part);
         if (nw == 0) break;
+        if (!kSplit) {
+            for (; cur_part < seg_part; ++cur_part) {
+                total = cur_part == 0 ? cur : total + cur;
+                cur = 0.0f;
+            }
+        }
         wave_sync();
         store_words(item, seg_base, nw, lmask, lwj0, gmask, gwj0, lane);
         BitWalk<kSeg> bw;
         bw.ne = sl.nonempty;
         const uint32_t steps = wave_max(sl.count);
-        if (stats)
-            acc = density_bits<true>(bw, steps, lmask, lwj0, lane, spm, pi, c, acc, cnt);
+        if (stats || kSplit)
+            cur = density_bits<true>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur, cnt);
         else
-            acc = density_bits<false>(bw, steps, lmask, lwj0, lane, spm, pi, c, acc, cnt);
+            cur = density_bits<false>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur, cnt);
         seg_base += nw;
         wave_sync();
         if (cu.done) break;
     }
-    if (lane == 0) gnw[item] = (uint32_t)seg_base;
+    if (kSplit) {
+        if (active) {
+            ps.acc[(size_t)part * ps.cap + s] = cur;
+            ps.cnt[(size_t)part * ps.cap + s] = cnt;
+        }
+        return;
+    }
+    for (; cur_part < kParts; ++cur_part) {
+        total = cur_part == 0 ? cur : total + cur;
+        cur = 0.0f;
+    }
+    if (active) density_epilogue(total, pi, s, c, svr, rows);
+    if (stats) add_nbr_stats(cnt, stats, lane);
+}
+
+// Finishes split density passes: per finite particle (sorted slots below the non-finite
+// bin), the three part sums in order, then the same epilogue as the single-wavefront pass.
+__global__ __launch_bounds__(256) void k_density_finish(
+    const float4* __restrict__ spm, float4* __restrict__ svr, ForceRow* __restrict__ rows,
+    const uint32_t* __restrict__ cs, StepConsts c, PartSums ps,
+    unsigned long long* __restrict__ stats) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nfin = cs[c.G];
+    const bool active = s < nfin;
+    uint32_t cnt = 0;
     if (active) {
-        const float rho = c.sigma * acc;
-        float4 v = svr[s];
-        v.w = rho;
-        svr[s] = v;
-        // neighbour rows of the force walk: B = m/rho and C = B P / 2, so that
-        // m_j (P_i + P_j) / (2 rho_j) = (P_i / 2) B_j + C_j
-        const float B = pi.w / rho;
-        const float P = c.k_gas * (rho - c.rho0);
-        rows[s].a = make_float4(pi.x, pi.y, pi.z, B);
-        rows[s].b = make_float4(v.x, v.y, v.z, 0.5f * B * P);
+        const float a0 = ps.acc[s], a1 = ps.acc[ps.cap + s], a2 = ps.acc[2 * ps.cap + s];
+        density_epilogue((a0 + a1) + a2, spm[s], s, c, svr, rows);
+        cnt = ps.cnt[s] + ps.cnt[ps.cap + s] + ps.cnt[2 * ps.cap + s];
     }
     if (stats) add_nbr_stats(cnt, stats, lane);
 }
@@ -234,12 +283,14 @@ __device__ __forceinline__ void integrate_clamp(float4& p, float3& v, float3 a, 
 
 __device__ __forceinline__ float3 final_accel(float3 fp, float3 fv, float m_i, float rho_i,
                                               const StepConsts& c) {
-    float fx = fp.x * c.grad_scale + fv.x * c.lap_scale;
-    float fy = fp.y * c.grad_scale + fv.y * c.lap_scale;
-    float fz = fp.z * c.grad_scale + fv.z * c.lap_scale;
-    float m = c.formulation == 1 ? rho_i : m_i;
-    float inv_m = 1.0f / m;
-    return make_float3(fx * inv_m, (fy + c.grav * m) * inv_m, fz * inv_m);
+    // explicit fused forms: left to contraction, the choice of which product to fuse can
+    // differ between the kernels that inline this, and so would their bits
+    const float fx = __builtin_fmaf(fp.x, c.grad_scale, fv.x * c.lap_scale);
+    const float fy = __builtin_fmaf(fp.y, c.grad_scale, fv.y * c.lap_scale);
+    const float fz = __builtin_fmaf(fp.z, c.grad_scale, fv.z * c.lap_scale);
+    const float m = c.formulation == 1 ? rho_i : m_i;
+    const float inv_m = 1.0f / m;
+    return make_float3(fx * inv_m, __builtin_fmaf(c.grav, m, fy) * inv_m, fz * inv_m);
 }
 
 // Pressure and viscosity sums of one target over neighbour rows (ForceRow a, b).
@@ -304,7 +355,50 @@ __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
     if (t + 1 < steps) f.add(pi, vi, p_half, a1, b1, c);
 }
 
-template <class Shape>
+// Folds the sums of finished parts into the running total, which starts at +0:
+// ((0 + part 0) + part 1) + part 2.  Closing `n` parts at once adds `f` for the first and
+// +0 for each further (empty) one, as the three-wavefront finish does.  The single-wavefront
+// walk keeps the total in LDS (`t`, six rows of 64 lanes): six more live registers would cost
+// it a wavefront per SIMD.
+__device__ __forceinline__ void fold_parts(float* __restrict__ t, uint32_t lane, ForceSums& f,
+                                           int n) {
+    if (n <= 0) return;
+    const float v[6] = {f.fp.x, f.fp.y, f.fp.z, f.fv.x, f.fv.y, f.fv.z};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        float x = t[k * 64 + lane] + v[k];
+        if (n >= 2) x = x + 0.0f;
+        t[k * 64 + lane] = x;
+    }
+    f = ForceSums();
+}
+
+// Acceleration, leapfrog, clamp and the caller-order writes of one target.
+__device__ __forceinline__ void force_epilogue(const ForceSums& f, float4 pi, float3 vi,
+                                               float rho_i, float P_i, uint32_t s,
+                                               const SceneGrid& g, const StepConsts& c,
+                                               const uint32_t* __restrict__ sid,
+                                               const float* __restrict__ dt_dev, float dt_arg,
+                                               float4* __restrict__ pm, float4* __restrict__ vel,
+                                               float4* __restrict__ acc_out,
+                                               float* __restrict__ rho_out,
+                                               float* __restrict__ prs_out) {
+    const float3 a = final_accel(f.fp, f.fv, pi.w, rho_i, c);
+    const float dt = dt_dev ? *dt_dev : dt_arg;
+    float3 v = vi;
+    float4 p = pi;
+    integrate_clamp(p, v, a, dt, c, g.lo, g.hi);
+    const uint32_t id = sid[s];
+    pm[id] = p;
+    vel[id] = make_float4(v.x, v.y, v.z, 0.0f);
+    acc_out[id] = make_float4(a.x, a.y, a.z, 0.0f);
+    rho_out[id] = rho_i;
+    prs_out[id] = P_i;
+}
+
+// Force pass over one chunk (kSplit false: all parts; true: the part of this wavefront,
+// three wavefronts per chunk, finished by k_force_finish).
+template <class Shape, bool kSplit>
 __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const uint2* __restrict__ sq,
@@ -312,7 +406,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sid,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     const uint32_t* __restrict__ gmask, const uint32_t* __restrict__ gwj0,
-    const uint32_t* __restrict__ gnw, const uint32_t* __restrict__ ggrp,
+    const uint32_t* __restrict__ gnw, const uint32_t* __restrict__ ggrp, PartSums ps,
     const float* __restrict__ dt_dev, float dt_arg,
     float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc_out, float* __restrict__ rho_out,
@@ -322,9 +416,13 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
     const uint32_t wave = uni(threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
+    __shared__ float ltot_all[kSplit ? 1 : Shape::waves][kSplit ? 1 : 6 * 64];
     uint32_t* lmask = lmask_all[wave];
     uint32_t* lwj0 = lwj0_all[wave];
-    const uint32_t item = swizzled_block() * Shape::waves + wave;
+    float* ltot = ltot_all[kSplit ? 0 : wave];
+    const uint32_t wid = swizzled_block() * Shape::waves + wave;
+    const uint32_t item = kSplit ? wid / kParts : wid;
+    const int part = kSplit ? (int)(wid % kParts) : -1;
     Chunk ch;
     uint32_t s;
     bool active;
@@ -332,24 +430,37 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
     const uint2 qi = active ? sq[s] : make_uint2(0u, 0u);
     init_sentinel<kSeg>(lmask, lwj0, lane, c);
+    if (!kSplit) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ltot[k * 64 + lane] = 0.0f;
+    }
     const float4 vr = active ? svr[s] : make_float4(0.f, 0.f, 0.f, 1.f);
     const float3 vi = make_float3(vr.x, vr.y, vr.z);
     const float rho_i = vr.w;
     const float P_i = c.k_gas * (rho_i - c.rho0);
     const float p_half = 0.5f * P_i;
     ForceSums f;
+    int cur_part = 0;
     const int total_words = (int)uni(gnw[item]);
     const bool stored = total_words <= kMaxWords;
     const uint32_t groups = stored ? uni(ggrp[item]) : 0u;
+    const int b1 = (int)(groups & 0xffu), b2 = (int)((groups >> 8) & 0xffu);
     Cursor cu;
-    if (!stored) cursor_begin(cu, ch, cs, c);
-    int seg_base = 0;
+    if (!stored) cursor_begin(cu, ch, cs, c, 0, part);
+    // stored words of this wavefront: all, or those of its part
+    int seg_base = 0, words_end = total_words;
+    if (kSplit) {
+        seg_base = part == 0 ? 0 : part == 1 ? b1 : b2;
+        words_end = part == 0 ? b1 : part == 1 ? b2 : total_words;
+    }
     for (;;) {
-        int nw;
+        int nw, seg_part;
         SegLane sl;
         if (stored) {
-            if (seg_base >= total_words) break;
-            nw = segment_end(seg_base, total_words, groups, kSeg - 1) - seg_base;
+            if (seg_base >= words_end) break;
+            seg_part = part_of_word(seg_base, groups);
+            const int pend = seg_part == 0 ? b1 : seg_part == 1 ? b2 : total_words;
+            nw = segment_end(seg_base, pend, groups, kSeg - 1) - seg_base;
             const size_t base = (size_t)item * kMaxWords + seg_base;
             sl.nonempty = 0;
             sl.count = 0;
@@ -361,8 +472,13 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
             }
             for (int w = (int)lane; w < nw; w += 64) lwj0[w] = gwj0[base + w];
         } else {
-            nw = phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane, sl);
+            nw = phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane, sl,
+                                      seg_part);
             if (nw == 0) break;
+        }
+        if (!kSplit) {
+            fold_parts(ltot, lane, f, seg_part - cur_part);
+            cur_part = seg_part;
         }
         wave_sync();
         BitWalk<kSeg> bw;
@@ -373,19 +489,56 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
         wave_sync();
         if (!stored && cu.done) break;
     }
+    if (kSplit) {
+        if (active) {
+            float4* out = ps.force + 2 * ((size_t)part * ps.cap + s);
+            out[0] = make_float4(f.fp.x, f.fp.y, f.fp.z, 0.0f);
+            out[1] = make_float4(f.fv.x, f.fv.y, f.fv.z, 0.0f);
+        }
+        return;
+    }
+    fold_parts(ltot, lane, f, kParts - cur_part);
     if (!active) return;
-    const float3 a = final_accel(f.fp, f.fv, pi.w, rho_i, c);
-    const float dt = dt_dev ? *dt_dev : dt_arg;
-    float3 v = vi;
-    float4 p = pi;
-    const SceneGrid& g = scenes[ch.scene];
-    integrate_clamp(p, v, a, dt, c, g.lo, g.hi);
+    ForceSums tot;
+    tot.fp = make_float3(ltot[0 * 64 + lane], ltot[1 * 64 + lane], ltot[2 * 64 + lane]);
+    tot.fv = make_float3(ltot[3 * 64 + lane], ltot[4 * 64 + lane], ltot[5 * 64 + lane]);
+    force_epilogue(tot, pi, vi, rho_i, P_i, s, scenes[ch.scene], c, sid, dt_dev, dt_arg, pm, vel,
+                   acc_out, rho_out, prs_out);
+}
+
+// Finishes split force passes: per finite particle, the three part sums in order, then the
+// same epilogue as the single-wavefront pass.
+__global__ __launch_bounds__(256) void k_force_finish(
+    const float4* __restrict__ spm, const float4* __restrict__ svr,
+    const uint32_t* __restrict__ sid, const uint32_t* __restrict__ cs, StepConsts c,
+    const SceneGrid* __restrict__ scenes, PartSums ps, const float* __restrict__ dt_dev,
+    float dt_arg, float4* __restrict__ pm, float4* __restrict__ vel,
+    float4* __restrict__ acc_out, float* __restrict__ rho_out, float* __restrict__ prs_out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= cs[c.G]) return;
+    // ((0 + part 0) + part 1) + part 2, as the single-wavefront walk folds them
+    float t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int p = 0; p < kParts; ++p) {
+        const float4* in = ps.force + 2 * ((size_t)p * ps.cap + s);
+        const float4 a = in[0], b = in[1];
+        t[0] = t[0] + a.x;
+        t[1] = t[1] + a.y;
+        t[2] = t[2] + a.z;
+        t[3] = t[3] + b.x;
+        t[4] = t[4] + b.y;
+        t[5] = t[5] + b.z;
+    }
+    ForceSums tot;
+    tot.fp = make_float3(t[0], t[1], t[2]);
+    tot.fv = make_float3(t[3], t[4], t[5]);
+    const float4 pi = spm[s];
+    const float4 vr = svr[s];
+    const float P_i = c.k_gas * (vr.w - c.rho0);
     const uint32_t id = sid[s];
-    pm[id] = p;
-    vel[id] = make_float4(v.x, v.y, v.z, 0.0f);
-    acc_out[id] = make_float4(a.x, a.y, a.z, 0.0f);
-    rho_out[id] = rho_i;
-    prs_out[id] = P_i;
+    force_epilogue(tot, pi, make_float3(vr.x, vr.y, vr.z), vr.w, P_i, s,
+                   scenes[scene_of_particle(scenes, c.nscenes, id)], c, sid, dt_dev, dt_arg, pm,
+                   vel, acc_out, rho_out, prs_out);
 }
 
 // Particles with a non-finite coordinate (dense index G) have no neighbour, not even
@@ -452,8 +605,9 @@ __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
     int64_t o = (pass == 1 && active) ? offs_by_id[sid[s]] : 0;
     for (;;) {
         SegLane sl;
+        int part;  // always 0 in the reference order
         const int nw =
-            phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane, sl);
+            phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane, sl, part);
         if (nw == 0) break;
         wave_sync();
         BitWalk<kSeg> bw;
@@ -488,27 +642,41 @@ const char* walk_shape_name(int v) {
         case 2: return "seg32_waves2";
         case 3: return "seg32_waves1";
         case 4: return "seg8_waves4";
+        case 5: return "seg16_waves4_split";
+        case 6: return "seg32_waves4_split";
         default: return "?";
     }
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The grid is
 // padded to a multiple of 8 and block b works on block (b % 8) * (grid / 8) + b / 8, so each
-// XCD walks one contiguous run of chunks, whose candidate columns overlap.
+// XCD walks one contiguous run of chunks, whose candidate columns overlap.  A split shape
+// runs three wavefronts per chunk (consecutive, so mostly in one workgroup).
 template <class Shape>
-static uint32_t grid_for(uint32_t chunks) {
-    uint32_t blocks = (chunks + Shape::waves - 1) / Shape::waves;
+static uint32_t grid_for(uint32_t chunks, bool split = false) {
+    const uint32_t waves = split ? chunks * (uint32_t)kParts : chunks;
+    uint32_t blocks = (waves + Shape::waves - 1) / Shape::waves;
     return (blocks + kXcds - 1) / kXcds * kXcds;
 }
 
+// CALL sees the shape as type S and the split flag as constexpr kS.
 #define SPHK_DISPATCH_SHAPE(v, CALL)     \
     switch (v) {                         \
-        case 0: { using S = Shape0; CALL; break; } \
-        case 1: { using S = Shape1; CALL; break; } \
-        case 2: { using S = Shape2; CALL; break; } \
-        case 3: { using S = Shape3; CALL; break; } \
-        default: { using S = Shape4; CALL; break; } \
+        case 0: { using S = Shape0; constexpr bool kS = false; CALL; break; } \
+        case 1: { using S = Shape1; constexpr bool kS = false; CALL; break; } \
+        case 2: { using S = Shape2; constexpr bool kS = false; CALL; break; } \
+        case 3: { using S = Shape3; constexpr bool kS = false; CALL; break; } \
+        case 5: { using S = Shape0; constexpr bool kS = true; CALL; break; } \
+        case 6: { using S = Shape1; constexpr bool kS = true; CALL; break; } \
+        default: { using S = Shape4; constexpr bool kS = false; CALL; break; } \
     }
+
+static hipError_t part_sums(const WalkBuffers& wb, uint32_t n, int shape, PartSums* ps) {
+    *ps = PartSums{wb.part_acc, wb.part_cnt, wb.part_force, wb.part_cap};
+    if (walk_shape_split(shape) && (!wb.part_acc || wb.part_cap < n))
+        return hipErrorInvalidValue;  // split shapes need per-part sums for every particle
+    return hipSuccess;
+}
 
 hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
                                float4* svr,
@@ -517,12 +685,18 @@ hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const u
                                const StepConsts& c, const SceneGrid* scenes,
                                unsigned long long* nbr_stats, int shape, hipStream_t st) {
     if (n == 0) return hipSuccess;
+    PartSums ps;
+    if (hipError_t err = part_sums(wb, n, shape, &ps)) return err;
     const uint32_t chunks = max_chunks(n, c);
-    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_density_walk<S>, dim3(grid_for<S>(chunks)),
+    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL((k_density_walk<S, kS>),
+                                                  dim3(grid_for<S>(chunks, kS)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
                                                   spm, sq, svr, rows, skey, cell_start, c,
                                                   scenes, wb.mask, wb.wj0, wb.nwords,
-                                                  wb.groups, nbr_stats));
+                                                  wb.groups, ps, nbr_stats));
+    if (walk_shape_split(shape))
+        hipLaunchKernelGGL(k_density_finish, dim3((n + 255) / 256), dim3(256), 0, st, spm, svr,
+                           rows, cell_start, c, ps, nbr_stats);
     return hipGetLastError();
 }
 
@@ -535,15 +709,21 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
                              float4* vel, float4* acc, float* rho, float* prs, int shape,
                              hipStream_t st) {
     if (n == 0) return hipSuccess;
+    PartSums ps;
+    if (hipError_t err = part_sums(wb, n, shape, &ps)) return err;
     const uint32_t chunks = max_chunks(n, c);
-    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_force_walk<S>, dim3(grid_for<S>(chunks)),
+    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL((k_force_walk<S, kS>),
+                                                  dim3(grid_for<S>(chunks, kS)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
                                                   spm, sq, svr, rows, skey, sid,
                                                   cell_start, c,
                                                   scenes,
-                                                  wb.mask, wb.wj0, wb.nwords, wb.groups, dt_dev,
-                                                  dt, pm, vel,
+                                                  wb.mask, wb.wj0, wb.nwords, wb.groups, ps,
+                                                  dt_dev, dt, pm, vel,
                                                   acc, rho, prs));
+    if (walk_shape_split(shape))
+        hipLaunchKernelGGL(k_force_finish, dim3((n + 255) / 256), dim3(256), 0, st, spm, svr,
+                           sid, cell_start, c, scenes, ps, dt_dev, dt, pm, vel, acc, rho, prs);
     // the non-finite bin holds at most n particles; the kernel reads its start on the device
     hipLaunchKernelGGL(k_nonfinite_step, dim3((n + 255) / 256), dim3(256), 0, st, spm,
                        const_cast<float4*>(svr), sid, cell_start, n, c, scenes, dt_dev, dt, pm, vel,
@@ -559,7 +739,9 @@ hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint
                             int shape, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const uint32_t chunks = max_chunks(n, c);
-    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL(k_nbr_lists<S>, dim3(grid_for<S>(chunks)),
+    // one wavefront per chunk in every shape (a split shape lists with its unsplit twin)
+    const int base = shape == 5 ? 0 : shape == 6 ? 1 : shape;
+    SPHK_DISPATCH_SHAPE(base, hipLaunchKernelGGL(k_nbr_lists<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
                                                   spm, sq, skey, sid, cell_start, c, scenes,
                                                   pass, count_by_id, offsets_by_id, idx));

@@ -168,9 +168,19 @@ __constant__ uint8_t kColumnOrder[2][25] = {
 // Balance group of position k of the balanced order.
 __device__ __forceinline__ int column_group(int k) { return k < 4 ? 0 : k < 8 ? 1 : k < 9 ? 2 : 3; }
 
+// Parts of a walk: the corner columns, the edge columns, the centre with the outer ring.
+// Sums over a chunk's neighbours are formed per part, each part in candidate order, and
+// combined as (part 0 + part 1) + part 2, so a walk split over three wavefronts (one part
+// each) gives the same bits as one wavefront walking all three.
+__device__ __forceinline__ int part_of_group(int g) { return g < 2 ? g : 2; }
+// First position of part p in the balanced order, and the end of it.
+__device__ __forceinline__ int part_begin(int p) { return p == 0 ? 0 : p == 1 ? 4 : 8; }
+__device__ __forceinline__ int part_end(int p) { return p == 0 ? 4 : p == 1 ? 8 : 25; }
+
 // Cursor over the candidate columns of a chunk.
 struct Cursor {
     int k;            // position in the column order
+    int kend;         // end of the positions walked
     int order;        // 0 balanced, 1 reference
     int group;        // balance group of the current column (0 in the reference order)
     bool fresh;       // the current column opens its group
@@ -204,25 +214,34 @@ __device__ __forceinline__ bool column_at(const Chunk& ch, int order, int k, int
 __device__ __forceinline__ void cursor_seek(Cursor& cu, const Chunk& ch,
                                             const uint32_t* __restrict__ cs, const StepConsts& c,
                                             int k) {
-    for (; k < 25; ++k) {
+    for (; k < cu.kend; ++k) {
         int dx, dy;
         if (!column_at(ch, cu.order, k, dx, dy)) continue;
+        uint32_t j, end;
+        column_range(ch, dx, dy, cs, c, j, end);
+        // an empty column holds no words and opens no group: the next non-empty column of
+        // its group does (segments and parts break where words of a new group start)
+        if (j >= end) continue;
         const int g = cu.order == 0 ? column_group(k) : 0;
-        cu.fresh = k == 0 || g != cu.group;
+        cu.fresh = g != cu.group;
         cu.k = k;
         cu.group = g;
-        column_range(ch, dx, dy, cs, c, cu.j, cu.end);
+        cu.j = j;
+        cu.end = end;
         return;
     }
-    cu.k = 25;
+    cu.k = cu.kend;
     cu.done = true;
 }
 
-// order 0: balanced (the SPH walks); 1: the reference's (neighbour-list export).
+// order 0: balanced (the SPH walks); 1: the reference's (neighbour-list export).  A part
+// p >= 0 (balanced order) limits the walk to the columns of that part; the word counts of
+// all groups are counted either way.
 __device__ __forceinline__ void cursor_begin(Cursor& cu, const Chunk& ch,
                                              const uint32_t* __restrict__ cs, const StepConsts& c,
-                                             int order = 0) {
+                                             int order = 0, int part = -1) {
     cu.order = order;
+    cu.kend = part >= 0 ? part_end(part) : 25;
     cu.done = false;
     cu.group = -1;
     cu.gw0 = cu.gw1 = cu.gw2 = cu.gw3 = 0;
@@ -240,7 +259,12 @@ __device__ __forceinline__ void cursor_begin(Cursor& cu, const Chunk& ch,
             cu.gw3 += g == 3 ? w : 0;
         }
     }
-    cursor_seek(cu, ch, cs, c, 0);
+    cursor_seek(cu, ch, cs, c, part >= 0 ? part_begin(part) : 0);
+}
+
+// Words of the chunk before part p (where its words start in the stored order).
+__device__ __forceinline__ int part_word_base(const Cursor& cu, int p) {
+    return p == 0 ? 0 : p == 1 ? cu.gw0 : cu.gw0 + cu.gw1;
 }
 
 // Moves to the next column when the current one is exhausted; false when all are.
@@ -259,8 +283,15 @@ __device__ __forceinline__ uint32_t pack_groups(const Cursor& cu) {
     return b1 | b2 << 8 | b3 << 16;
 }
 
-// End of the stored-word segment that starts at word `pos`: the last group start (or the
-// end) within `cap` words, or pos + cap when the group at pos is larger than that.
+// Part of the stored word at `pos` (from the packed group starts).
+__device__ __forceinline__ int part_of_word(int pos, uint32_t groups) {
+    const int b1 = (int)(groups & 0xffu), b2 = (int)((groups >> 8) & 0xffu);
+    return pos < b1 ? 0 : pos < b2 ? 1 : 2;
+}
+
+// End of the stored-word segment that starts at word `pos`: never past the end of its part
+// (`total` is that end); the last group start (or that end) within `cap` words, or pos + cap
+// when the group at pos is larger than that.
 __device__ __forceinline__ int segment_end(int pos, int total, uint32_t groups, int cap) {
     const int lim = min(pos + cap, total);
     if (lim == total) return total;
@@ -318,15 +349,19 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
                                               const StepConsts& c, uint2 qi, bool active,
                                               uint32_t* __restrict__ lmask,
                                               uint32_t* __restrict__ lwj0, uint32_t lane,
-                                              SegLane& sl) {
+                                              SegLane& sl, int& part) {
     static_assert(kSeg <= 32, "one nonempty bit per word");
     int w = 0;
     sl.nonempty = 0;
     sl.count = 0;
+    part = 0;
     while (w < kSeg - 1 && cursor_settle(cu, ch, cs, c)) {
+        const int p = part_of_group(cu.group);
+        if (w == 0) part = p;
         if (cu.fresh) {
-            // a group that does not fit in what is left of the segment starts the next one
-            if (w > 0 && w + cu.group_words() > kSeg - 1) break;
+            // a new part, or a group that does not fit in what is left of the segment,
+            // starts the next one
+            if (w > 0 && (p != part || w + cu.group_words() > kSeg - 1)) break;
             cu.fresh = false;
         }
         const uint32_t j0 = uni(cu.j);

@@ -394,34 +394,52 @@ def test_config1_fullsize_step0(sphk, oracle, scenes):
 
 def test_walk_shapes_bitexact(sphk, scenes):
     # every launch shape of the walks gives bit-identical results (same per-particle order)
-    s = scenes.config0(seed=21)
-    ref = None
-    for v in range(len(sphk.walk_shape_names())):
-        e = _engine(sphk, s)
-        e.set_walk_shapes(v, v)
-        e.step(s.dt)
-        out = (e.positions(), e.velocities(), e.densities(), e.accelerations())
-        if ref is None:
-            ref = out
-        else:
-            for a, b in zip(ref, out):
-                np.testing.assert_array_equal(a, b)
-        offs, idx = e.neighbors()
-        if v == 0:
-            ref_n = (offs, idx)
-        else:
-            np.testing.assert_array_equal(offs, ref_n[0])
-            np.testing.assert_array_equal(idx, ref_n[1])
+    # (the split shapes walk a chunk with three wavefronts and sum the parts afterwards; they
+    # mix freely with the single-wavefront ones, density and force chosen independently)
+    # Both scenes leave candidate columns empty at the block's faces (an empty column must
+    # not hide the start of the next column group from the segment and part breaks).
+    names = sphk.walk_shape_names()
+    split = [v for v, nm in enumerate(names) if nm.endswith("_split")]
+    assert split, names
+    pairs = [(v, v) for v in range(len(names))]
+    pairs += [(split[0], 0), (0, split[0]), (split[-1], 2)]
+    for s, dt in ((scenes.config0(seed=21), None), (scenes.small_block(10, 10, 10, seed=8), 1e-4)):
+        dt = dt or s.dt
+        ref = None
+        for d, f in pairs:
+            print(f"{s.name}: walk shapes density={names[d]} force={names[f]}", flush=True)
+            e = _engine(sphk, s)
+            e.set_walk_shapes(d, f)
+            for _ in range(3):
+                e.step(dt)
+            out = (e.positions(), e.velocities(), e.densities(), e.accelerations())
+            if ref is None:
+                ref = out
+            else:
+                for a, b in zip(ref, out):
+                    np.testing.assert_array_equal(a, b)
+            offs, idx = e.neighbors()
+            if d == 0 and f == 0:
+                ref_n = (offs, idx)
+            else:
+                np.testing.assert_array_equal(offs, ref_n[0])
+                np.testing.assert_array_equal(idx, ref_n[1])
 
 
-def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes):
+@pytest.mark.parametrize("split", [False, True])
+def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes, split):
     # a pile-up cell with more than 4096 candidates per chunk exceeds the stored-word budget;
-    # the force pass then re-tests.  Both paths must agree with the oracle.
+    # the force pass then re-tests.  Both paths must agree with the oracle, in the
+    # single-wavefront and the three-wavefront (split) walks.
     rng = np.random.default_rng(3)
     pos = np.concatenate([rng.uniform(0.1, 0.13, size=(3000, 3)),
                           rng.uniform(0.3, 0.5, size=(500, 3))]).astype(np.float32)
     p = sphk.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
     e = sphk.Engine(len(pos), p)
+    if split:
+        names = sphk.walk_shape_names()
+        v = names.index("seg16_waves4_split")
+        e.set_walk_shapes(v, v)
     e.upload(pos)
     o = oracle.Oracle(len(pos), oracle.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0,
                                                       zmax=1))
