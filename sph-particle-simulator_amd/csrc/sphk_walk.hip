@@ -444,14 +444,13 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     const int total_words = (int)uni(gnw[item]);
     const bool stored = total_words <= kMaxWords;
     const uint32_t groups = stored ? uni(ggrp[item]) : 0u;
-    const int b1 = (int)(groups & 0xffu), b2 = (int)((groups >> 8) & 0xffu);
     Cursor cu;
     if (!stored) cursor_begin(cu, ch, cs, c, 0, part);
     // stored words of this wavefront: all, or those of its part
     int seg_base = 0, words_end = total_words;
     if (kSplit) {
-        seg_base = part == 0 ? 0 : part == 1 ? b1 : b2;
-        words_end = part == 0 ? b1 : part == 1 ? b2 : total_words;
+        seg_base = part_word_start(groups, part);
+        words_end = part_word_end(groups, part, total_words);
     }
     for (;;) {
         int nw, seg_part;
@@ -459,7 +458,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
         if (stored) {
             if (seg_base >= words_end) break;
             seg_part = part_of_word(seg_base, groups);
-            const int pend = seg_part == 0 ? b1 : seg_part == 1 ? b2 : total_words;
+            const int pend = part_word_end(groups, seg_part, total_words);
             nw = segment_end(seg_base, pend, groups, kSeg - 1) - seg_base;
             const size_t base = (size_t)item * kMaxWords + seg_base;
             sl.nonempty = 0;

@@ -155,27 +155,40 @@ struct Chunk {
 };
 
 // Candidate columns in walk order, as (dx + 2) * 5 + (dy + 2).  Row 0, balanced order: the
-// four corner columns, the four edge columns, the centre, then the ring two columns out
+// four corner columns, the centre, the four edge columns, then the ring two columns out
 // (walked only when axis_edge_flags asks for it).  A lane near one side of the chunk's
 // column finds about as many neighbours in each of the first three groups as a lane near
 // the other side, so segments that hold whole groups have even per-lane bit counts (the
 // wavefront runs the largest).  Row 1: the reference's (dx, dy) lexicographic order
 // (SpatialHash::query_squared), for the neighbour-list export.
 __constant__ uint8_t kColumnOrder[2][25] = {
-    {6, 18, 8, 16, 7, 17, 11, 13, 12, 0, 1, 2, 3, 4, 5, 9, 10, 14, 15, 19, 20, 21, 22, 23, 24},
+    {6, 18, 8, 16, 12, 7, 17, 11, 13, 0, 1, 2, 3, 4, 5, 9, 10, 14, 15, 19, 20, 21, 22, 23, 24},
     {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24}};
 
-// Balance group of position k of the balanced order.
-__device__ __forceinline__ int column_group(int k) { return k < 4 ? 0 : k < 8 ? 1 : k < 9 ? 2 : 3; }
+// Balance group of position k of the balanced order, and the first position of group g.
+__device__ __forceinline__ int column_group(int k) { return k < 4 ? 0 : k < 5 ? 1 : k < 9 ? 2 : 3; }
+__device__ __forceinline__ int group_begin(int g) { return g == 0 ? 0 : g == 1 ? 4 : g == 2 ? 5 : 9; }
 
-// Parts of a walk: the corner columns, the edge columns, the centre with the outer ring.
+// Parts of a walk: the corner columns with the centre, the edge columns, the outer ring.
 // Sums over a chunk's neighbours are formed per part, each part in candidate order, and
 // combined as (part 0 + part 1) + part 2, so a walk split over three wavefronts (one part
-// each) gives the same bits as one wavefront walking all three.
-__device__ __forceinline__ int part_of_group(int g) { return g < 2 ? g : 2; }
+// each) gives the same bits as one wavefront walking all three.  Parts are runs of whole
+// groups: part p starts at group kPartGroup1 (p = 1) or kPartGroup2 (p = 2).  Part breaks
+// are segment breaks of the single-wavefront walk too; these parts put them where whole
+// groups end anyway in dense blocks (about 8 words per column), so that walk keeps the
+// segments it would have without parts, while the split walk runs the corners with the
+// centre and the edges side by side (the ring is rarely walked).
+constexpr int kPartGroup1 = 2;
+constexpr int kPartGroup2 = 3;
+__device__ __forceinline__ int part_of_group(int g) {
+    return g < kPartGroup1 ? 0 : g < kPartGroup2 ? 1 : 2;
+}
+__device__ __forceinline__ int part_first_group(int p) {
+    return p == 0 ? 0 : p == 1 ? kPartGroup1 : kPartGroup2;
+}
 // First position of part p in the balanced order, and the end of it.
-__device__ __forceinline__ int part_begin(int p) { return p == 0 ? 0 : p == 1 ? 4 : 8; }
-__device__ __forceinline__ int part_end(int p) { return p == 0 ? 4 : p == 1 ? 8 : 25; }
+__device__ __forceinline__ int part_begin(int p) { return group_begin(part_first_group(p)); }
+__device__ __forceinline__ int part_end(int p) { return p == 2 ? 25 : part_begin(p + 1); }
 
 // Cursor over the candidate columns of a chunk.
 struct Cursor {
@@ -264,7 +277,8 @@ __device__ __forceinline__ void cursor_begin(Cursor& cu, const Chunk& ch,
 
 // Words of the chunk before part p (where its words start in the stored order).
 __device__ __forceinline__ int part_word_base(const Cursor& cu, int p) {
-    return p == 0 ? 0 : p == 1 ? cu.gw0 : cu.gw0 + cu.gw1;
+    const int g = part_first_group(p);
+    return (g > 0 ? cu.gw0 : 0) + (g > 1 ? cu.gw1 : 0) + (g > 2 ? cu.gw2 : 0);
 }
 
 // Moves to the next column when the current one is exhausted; false when all are.
@@ -283,10 +297,20 @@ __device__ __forceinline__ uint32_t pack_groups(const Cursor& cu) {
     return b1 | b2 << 8 | b3 << 16;
 }
 
-// Part of the stored word at `pos` (from the packed group starts).
+// First stored word of group g (from the packed group starts).
+__device__ __forceinline__ int group_word_start(uint32_t groups, int g) {
+    return g == 0 ? 0 : (int)((groups >> (8 * (g - 1))) & 0xffu);
+}
+// First stored word of part p, and the end of its words.
+__device__ __forceinline__ int part_word_start(uint32_t groups, int p) {
+    return group_word_start(groups, part_first_group(p));
+}
+__device__ __forceinline__ int part_word_end(uint32_t groups, int p, int total) {
+    return p == 2 ? total : part_word_start(groups, p + 1);
+}
+// Part of the stored word at `pos`.
 __device__ __forceinline__ int part_of_word(int pos, uint32_t groups) {
-    const int b1 = (int)(groups & 0xffu), b2 = (int)((groups >> 8) & 0xffu);
-    return pos < b1 ? 0 : pos < b2 ? 1 : 2;
+    return pos < part_word_start(groups, 1) ? 0 : pos < part_word_start(groups, 2) ? 1 : 2;
 }
 
 // End of the stored-word segment that starts at word `pos`: never past the end of its part
