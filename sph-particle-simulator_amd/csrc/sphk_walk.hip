@@ -162,6 +162,8 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
     const uint2 qi = active ? sq[s] : make_uint2(0u, 0u);
     init_sentinel<kSeg>(lmask, lwj0, lane, c);
+    MfmaFilter mf;
+    mfma_filter_begin(mf, ch, sq, qi, lane, c.qneg);
     Cursor cu;
     cursor_begin(cu, ch, cs, c, 0, part);
     if (lane == 0 && part <= 0) {
@@ -176,8 +178,8 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     for (;;) {
         SegLane sl;
         int seg_part;
-        const int nw = phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane,
-                                            sl, seg_part);
+        const int nw = phase1_segment<kSeg, true>(cu, ch, sq, cs, c, qi, active, lmask, lwj0,
+                                                  lane, sl, seg_part, &mf);
         if (nw == 0) break;
         if (!kSplit) {
             for (; cur_part < seg_part; ++cur_part) {

@@ -152,6 +152,7 @@ struct Chunk {
     int rx, ry;           // column (grid-relative)
     int dx0, dx1, dy0, dy1;
     int zlo, zhi;         // candidate z cells (inclusive), grid-relative
+    int zfirst;           // z cell of the first particle, grid-relative
 };
 
 // Candidate columns in walk order, as (dx + 2) * 5 + (dy + 2).  Row 0, balanced order: the
@@ -366,14 +367,115 @@ __device__ __forceinline__ uint32_t push_quant(uint32_t word, uint2 a, uint2 b, 
 // Candidate k of a word sits in bit 31-k (words of fewer than 32 candidates are
 // left-aligned).  Word kSeg - 1 is the sentinel of BitWalk and is never written here.
 // Returns the number of words written.
-template <int kSeg>
+// ---------------------------------------------------------------------------
+// The same quantized test on the matrix cores.  With a and b the target's and the
+// candidate's quantized coordinates relative to an origin near the chunk,
+//   |a - b|^2 - (T + 1) = b.b - 2 a.b + (a.a - T - 1),
+// one v_mfma_f32_32x32x16_f16 evaluates it for 32 candidates (rows) x 32 targets (columns):
+// K slots 0-2 hold b and -2a, slots 3-4 b.b as lo + 2048 hi against (1, 2048), slots 5-6
+// (1, 2048) against a.a - T - 1 as lo + 2048 hi, the rest zero.  Every operand is an integer of
+// magnitude <= 2048 (exact in f16) and every partial sum an integer below 2^24 (exact in the
+// FP32 accumulator) when all coordinates lie within +-1024 of the origin: the first target's
+// x, y (candidates are at most 2 cells + 1 unit away: |.| <= 769) and, in z, the middle of the
+// candidate cells (|.| <= 128 (zhi - zlo + 1) + 257: at most 5 candidate z cells).  Chunks with
+// more use the VALU test.  The verdicts equal the VALU test's wherever that one does not wrap
+// around, which only adds far candidates; either way only such candidates differ, and they
+// add exactly zero.
+// ---------------------------------------------------------------------------
+typedef _Float16 sphk_half8 __attribute__((ext_vector_type(8)));
+typedef float sphk_f32x16 __attribute__((ext_vector_type(16)));
+
+struct MfmaFilter {
+    bool on;             // uniform: the chunk fits the exact range
+    uint32_t ox, oz;     // origin: packed x | y << 16 (16-bit wrap-around), z
+    sphk_half8 b1, b2;   // B operands: targets 0-31 and 32-63 (lanes 0-31 hold them)
+    uint32_t crow;       // candidate of this lane's A row: 16 ((m >> 2) & 1) + 4 (m >> 3) + (m & 3)
+};
+
+__device__ __forceinline__ void mfma_filter_begin(MfmaFilter& mf, const Chunk& ch,
+                                                  const uint2* __restrict__ sq, uint2 qi,
+                                                  uint32_t lane, int32_t qneg) {
+    mf.on = ch.zhi - ch.zlo <= 4;
+    const uint32_t qfx = uni(sq[ch.s0].x), qfz = uni(sq[ch.s0].y);
+    mf.ox = qfx;
+    mf.oz = qfz + (uint32_t)(128 * (ch.zlo + ch.zhi + 1 - 2 * ch.zfirst));
+    const sphk_short2 a = __builtin_bit_cast(sphk_short2, qi.x) - __builtin_bit_cast(sphk_short2, mf.ox);
+    // (lanes without a target compute garbage in their own columns only; unsigned
+    // arithmetic keeps that free of overflow)
+    const uint32_t az = qi.y - mf.oz;
+    const int32_t aa = __builtin_amdgcn_sdot2(a, a, (int32_t)(az * az + (uint32_t)qneg), false);
+    sphk_half8 p;
+    p[0] = (_Float16)(-2 * (int32_t)a.x);
+    p[1] = (_Float16)(-2 * (int32_t)a.y);
+    p[2] = (_Float16)(int32_t)(0u - 2u * az);
+    p[3] = (_Float16)1.0f;
+    p[4] = (_Float16)2048.0f;
+    p[5] = (_Float16)(aa & 2047);
+    p[6] = (_Float16)(aa >> 11);
+    p[7] = (_Float16)0.0f;
+    sphk_half8 q;
+    const uint4 pu = __builtin_bit_cast(uint4, p);
+    uint4 qu;
+    qu.x = (uint32_t)__shfl_xor((int)pu.x, 32);
+    qu.y = (uint32_t)__shfl_xor((int)pu.y, 32);
+    qu.z = (uint32_t)__shfl_xor((int)pu.z, 32);
+    qu.w = (uint32_t)__shfl_xor((int)pu.w, 32);
+    q = __builtin_bit_cast(sphk_half8, qu);
+    const bool lo = lane < 32u;
+    const sphk_half8 zero = {};
+    mf.b1 = lo ? p : zero;
+    mf.b2 = lo ? q : zero;
+    const uint32_t m = lane & 31u;
+    mf.crow = 16u * ((m >> 2) & 1u) + 4u * (m >> 3) + (m & 3u);
+}
+
+// Verdict word of this lane's target over candidates j0 .. j0 + cnt - 1 (cnt in 1..32),
+// candidate k at bit 31 - k, like the VALU test.
+__device__ __forceinline__ uint32_t mfma_filter_word(const MfmaFilter& mf,
+                                                     const uint2* __restrict__ sq, uint32_t j0,
+                                                     uint32_t cnt, uint32_t lane) {
+    const uint2 qb = sq[j0 + min(mf.crow, cnt - 1u)];
+    const sphk_short2 b = __builtin_bit_cast(sphk_short2, qb.x) - __builtin_bit_cast(sphk_short2, mf.ox);
+    const uint32_t bzu = qb.y - mf.oz;
+    const int32_t bz = (int32_t)bzu;
+    const int32_t bb = __builtin_amdgcn_sdot2(b, b, (int32_t)(bzu * bzu), false);
+    sphk_half8 a;
+    a[0] = (_Float16)(int32_t)b.x;
+    a[1] = (_Float16)(int32_t)b.y;
+    a[2] = (_Float16)bz;
+    a[3] = (_Float16)(bb & 2047);
+    a[4] = (_Float16)(bb >> 11);
+    a[5] = (_Float16)1.0f;
+    a[6] = (_Float16)2048.0f;
+    a[7] = (_Float16)0.0f;
+    const bool lo = lane < 32u;
+    const sphk_half8 zero = {};
+    a = lo ? a : zero;
+    const sphk_f32x16 z16 = {};
+    const sphk_f32x16 d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, mf.b1, z16, 0, 0, 0);
+    const sphk_f32x16 d2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, mf.b2, z16, 0, 0, 0);
+    uint32_t x = 0, y = 0;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        x = __builtin_amdgcn_alignbit(x, __float_as_uint(d1[v]), 31);
+        y = __builtin_amdgcn_alignbit(y, __float_as_uint(d2[v]), 31);
+    }
+    // lanes 0-31 own targets 0-31 (candidates 0-15 here, 16-31 in lane + 32's x); lanes 32-63
+    // own targets 32-63 (candidates 16-31 here, 0-15 in lane - 32's y)
+    const uint32_t recv = (uint32_t)__shfl_xor((int)(lo ? y : x), 32);
+    const uint32_t word = lo ? (x << 16 | recv) : (recv << 16 | y);
+    return word & (~0u << (32u - cnt));
+}
+
+template <int kSeg, bool kMfma = false>
 __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
                                               const uint2* __restrict__ sq,
                                               const uint32_t* __restrict__ cs,
                                               const StepConsts& c, uint2 qi, bool active,
                                               uint32_t* __restrict__ lmask,
                                               uint32_t* __restrict__ lwj0, uint32_t lane,
-                                              SegLane& sl, int& part) {
+                                              SegLane& sl, int& part,
+                                              const MfmaFilter* mf = nullptr) {
     static_assert(kSeg <= 32, "one nonempty bit per word");
     int w = 0;
     sl.nonempty = 0;
@@ -392,7 +494,9 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
         const uint32_t cnt = uni(min(32u, cu.end - j0));
         uint32_t word = 0;
         const uint2* __restrict__ pb = sq + j0;  // uniform base: scalar loads
-        if (cnt == 32u) {
+        if (kMfma && mf->on) {
+            word = mfma_filter_word(*mf, sq, j0, cnt, lane);
+        } else if (cnt == 32u) {
 #pragma unroll
             for (int k = 0; k < 32; ++k) word = push_quant(word, qi, pb[k], c.qneg);
         } else {
@@ -453,6 +557,7 @@ __device__ __forceinline__ bool load_chunk(uint32_t item, const uint32_t* __rest
     ch.dy1 = unii(1 + fyh);
     ch.zlo = unii(zmin - 1 - fzl);
     ch.zhi = unii(zmax + 1 + fzh);
+    ch.zfirst = zmin;
     return true;
 }
 
