@@ -390,6 +390,8 @@ struct MfmaFilter {
     uint32_t ox, oz;     // origin: packed x | y << 16 (16-bit wrap-around), z
     sphk_half8 b1, b2;   // B operands: targets 0-31 and 32-63 (lanes 0-31 hold them)
     uint32_t crow;       // candidate of this lane's A row: 16 ((m >> 2) & 1) + 4 (m >> 3) + (m & 3)
+    uint32_t pref_j;     // uniform: first slot of the word whose row is in `pref` (~0: none)
+    uint2 pref;          // this lane's row of the next word of the current column
 };
 
 __device__ __forceinline__ void mfma_filter_begin(MfmaFilter& mf, const Chunk& ch,
@@ -427,14 +429,17 @@ __device__ __forceinline__ void mfma_filter_begin(MfmaFilter& mf, const Chunk& c
     mf.b2 = lo ? q : zero;
     const uint32_t m = lane & 31u;
     mf.crow = 16u * ((m >> 2) & 1u) + 4u * (m >> 3) + (m & 3u);
+    mf.pref_j = ~0u;
+    mf.pref = make_uint2(0u, 0u);
 }
 
 // Verdict word of this lane's target over candidates j0 .. j0 + cnt - 1 (cnt in 1..32),
-// candidate k at bit 31 - k, like the VALU test.
-__device__ __forceinline__ uint32_t mfma_filter_word(const MfmaFilter& mf,
+// candidate k at bit 31 - k, like the VALU test.  The rows of the column's next word (which
+// ends at `end`) are requested before this word's arithmetic.
+__device__ __forceinline__ uint32_t mfma_filter_word(MfmaFilter& mf,
                                                      const uint2* __restrict__ sq, uint32_t j0,
-                                                     uint32_t cnt, uint32_t lane) {
-    const uint2 qb = sq[j0 + min(mf.crow, cnt - 1u)];
+                                                     uint32_t cnt, uint32_t end, uint32_t lane) {
+    const uint2 qb = mf.pref_j == j0 ? mf.pref : sq[j0 + min(mf.crow, cnt - 1u)];
     const sphk_short2 b = __builtin_bit_cast(sphk_short2, qb.x) - __builtin_bit_cast(sphk_short2, mf.ox);
     const uint32_t bzu = qb.y - mf.oz;
     const int32_t bz = (int32_t)bzu;
@@ -451,6 +456,14 @@ __device__ __forceinline__ uint32_t mfma_filter_word(const MfmaFilter& mf,
     const bool lo = lane < 32u;
     const sphk_half8 zero = {};
     a = lo ? a : zero;
+    // next word's rows, requested once this word's are used up (the empty asm pins the A
+    // operand before the request, so this word does not wait for the next one's load)
+    asm volatile("" ::"v"(a));
+    const uint32_t jn = j0 + cnt;
+    if (jn < end) {
+        mf.pref = sq[jn + min(mf.crow, min(32u, end - jn) - 1u)];
+        mf.pref_j = jn;
+    }
     const sphk_f32x16 z16 = {};
     const sphk_f32x16 d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, mf.b1, z16, 0, 0, 0);
     const sphk_f32x16 d2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, mf.b2, z16, 0, 0, 0);
@@ -475,7 +488,7 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
                                               uint32_t* __restrict__ lmask,
                                               uint32_t* __restrict__ lwj0, uint32_t lane,
                                               SegLane& sl, int& part,
-                                              const MfmaFilter* mf = nullptr) {
+                                              MfmaFilter* mf = nullptr) {
     static_assert(kSeg <= 32, "one nonempty bit per word");
     int w = 0;
     sl.nonempty = 0;
@@ -495,7 +508,7 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
         uint32_t word = 0;
         const uint2* __restrict__ pb = sq + j0;  // uniform base: scalar loads
         if (kMfma && mf->on) {
-            word = mfma_filter_word(*mf, sq, j0, cnt, lane);
+            word = mfma_filter_word(*mf, sq, j0, cnt, cu.end, lane);
         } else if (cnt == 32u) {
 #pragma unroll
             for (int k = 0; k < 32; ++k) word = push_quant(word, qi, pb[k], c.qneg);
