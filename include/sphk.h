@@ -86,7 +86,9 @@ typedef struct {
 /* Default parameter values of SPHParameters{} (src/sph_engine.h:14-34). */
 int sphk_default_params(sphk_params* out);
 
-/* Lifecycle.  Replaces SPHEngine(size_t max_particles) (src/sph_engine.cpp:13-18). */
+/* Lifecycle.  Replaces SPHEngine(size_t max_particles) (src/sph_engine.cpp:13-18).
+ * capacity: 1 .. 2^27 - 2 particles (the walks address neighbour rows of 32 bytes with
+ * 32-bit byte offsets); larger requests fail with SPHK_E_INVALID. */
 int sphk_create(size_t capacity, sphk_engine** out);
 int sphk_destroy(sphk_engine* e);
 

@@ -585,7 +585,9 @@ int sphk_create(size_t capacity, sphk_engine** out) {
     if (!out) return set_error(SPHK_E_INVALID, "null output handle");
     *out = nullptr;
     if (capacity == 0) return set_error(SPHK_E_INVALID, "capacity must be positive");
-    if (capacity > 0x7fffffffull) return set_error(SPHK_E_INVALID, "capacity above 2^31-1");
+    // the walks address neighbour rows (32 bytes, capacity + 1 of them) by 32-bit byte offsets
+    if (capacity > kMaxCapacity)
+        return set_error(SPHK_E_INVALID, "capacity above %llu", (unsigned long long)kMaxCapacity);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_error(SPHK_E_NODEVICE, "no HIP device visible");

@@ -72,6 +72,10 @@ constexpr int kRadixItems = 8;  // keys per thread
 constexpr int kRadixTile = kRadixThreads * kRadixItems;
 constexpr int kRadixMaxBits = 11;
 
+// Largest engine capacity: (capacity + 1) rows of 32 bytes stay below 2^32 bytes, the range
+// of the walks' 32-bit row offsets.
+constexpr uint64_t kMaxCapacity = (1ull << 27) - 2;
+
 // neighbour walks: words of 32 candidates kept in LDS per wave, and per chunk in HBM
 constexpr int kMaxWords = 128;
 constexpr int kNumWalkShapes = 7;  // launch shapes selectable per walk (see walk_shape_name)

@@ -104,15 +104,15 @@ __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                               const float4* __restrict__ spm, const float4& pi,
                                               const StepConsts& c, float acc, uint32_t& cnt) {
     if (steps == 0) return acc;
-    float4 q0 = spm[bw.next(lmask, lwj0, lane)];
-    float4 q1 = spm[bw.next(lmask, lwj0, lane)];
+    float4 q0 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
+    float4 q1 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
     uint32_t t = 0;
     for (; t + 3 <= steps; t += 3) {
-        const float4 q2 = spm[bw.next(lmask, lwj0, lane)];
+        const float4 q2 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
         acc = density_pair<kCount>(pi, q0, c, acc, cnt);
-        q0 = spm[bw.next(lmask, lwj0, lane)];
+        q0 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
         acc = density_pair<kCount>(pi, q1, c, acc, cnt);
-        q1 = spm[bw.next(lmask, lwj0, lane)];
+        q1 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
         acc = density_pair<kCount>(pi, q2, c, acc, cnt);
     }
     if (t < steps) acc = density_pair<kCount>(pi, q0, c, acc, cnt);
@@ -335,22 +335,23 @@ __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                            const float3& vi, float p_half, const StepConsts& c,
                                            ForceSums& f) {
     if (steps == 0) return;
-    uint32_t j = bw.next(lmask, lwj0, lane);
-    float4 a0 = rows[j].a, b0 = rows[j].b;
-    j = bw.next(lmask, lwj0, lane);
-    float4 a1 = rows[j].a, b1 = rows[j].b;
+    static_assert(sizeof(ForceRow) == 32, "rows are 32-byte blocks");
+    uint32_t o = bw.template next<5>(lmask, lwj0, lane);
+    float4 a0 = row_at(rows, o).a, b0 = row_at(rows, o).b;
+    o = bw.template next<5>(lmask, lwj0, lane);
+    float4 a1 = row_at(rows, o).a, b1 = row_at(rows, o).b;
     uint32_t t = 0;
     for (; t + 3 <= steps; t += 3) {
-        j = bw.next(lmask, lwj0, lane);
-        const float4 a2 = rows[j].a, b2 = rows[j].b;
+        o = bw.template next<5>(lmask, lwj0, lane);
+        const float4 a2 = row_at(rows, o).a, b2 = row_at(rows, o).b;
         f.add(pi, vi, p_half, a0, b0, c);
-        j = bw.next(lmask, lwj0, lane);
-        a0 = rows[j].a;
-        b0 = rows[j].b;
+        o = bw.template next<5>(lmask, lwj0, lane);
+        a0 = row_at(rows, o).a;
+        b0 = row_at(rows, o).b;
         f.add(pi, vi, p_half, a1, b1, c);
-        j = bw.next(lmask, lwj0, lane);
-        a1 = rows[j].a;
-        b1 = rows[j].b;
+        o = bw.template next<5>(lmask, lwj0, lane);
+        a1 = row_at(rows, o).a;
+        b1 = row_at(rows, o).b;
         f.add(pi, vi, p_half, a2, b2, c);
     }
     if (t < steps) f.add(pi, vi, p_half, a0, b0, c);

@@ -582,6 +582,9 @@ __device__ __forceinline__ bool load_chunk(uint32_t item, const uint32_t* __rest
 template <int kSeg>
 struct BitWalk {
     uint32_t m = 0, jb = 0, ne = 0;
+    // next candidate: its slot (kShift 0) or its byte offset in rows of 2^kShift bytes, so
+    // a gather needs one 32-bit add-shift and a load with the row base in SGPRs
+    template <int kShift = 0>
     __device__ __forceinline__ uint32_t next(const uint32_t* __restrict__ lmask,
                                              const uint32_t* __restrict__ lwj0, uint32_t lane) {
         const uint32_t w = (uint32_t)__builtin_ctz(ne | (1u << (kSeg - 1)));
@@ -589,13 +592,19 @@ struct BitWalk {
         const uint32_t jn = lwj0[w];
         const bool need = m == 0u;
         m = need ? mn : m;
-        jb = need ? jn : jb;
+        jb = need ? (jn << kShift) : jb;
         ne = need ? (ne & (ne - 1u)) : ne;
         const uint32_t k = (uint32_t)__builtin_clz(m);
         m ^= 0x80000000u >> k;
-        return jb + k;
+        return jb + (k << kShift);
     }
 };
+
+// Row at byte offset `off` (below 2^32) of an array in global memory.
+template <class T>
+__device__ __forceinline__ const T& row_at(const T* __restrict__ base, uint32_t off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
 
 // Writes the sentinel word kSeg - 1 of a wavefront's segment buffers (once per wavefront;
 // phase 1 and the stored-word loads never overwrite it).

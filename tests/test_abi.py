@@ -47,3 +47,13 @@ def test_create_without_gpu_fails_loudly(sphk):
     with pytest.raises(sphk.SphkError) as ex:
         sphk.Engine(16)
     assert ex.value.code == -5
+
+
+def test_capacity_limit_is_checked_before_the_device(sphk):
+    # rows are addressed by 32-bit byte offsets: capacity + 1 rows of 32 bytes below 2^32
+    with pytest.raises(sphk.SphkError) as ex:
+        sphk.Engine((1 << 27) - 1)
+    assert ex.value.code == -1
+    with pytest.raises(sphk.SphkError) as ex:
+        sphk.Engine(0)
+    assert ex.value.code == -1
