@@ -7,15 +7,16 @@
 // to +-2 on an axis when `axis_edge_flags` proves a pair two cells apart possible for some
 // lane of the chunk).  Each column contributes one contiguous range of sorted slots.
 //
-// Phase 1 (test): candidates are walked one at a time for the whole wave, column by column
-//   (corner columns, edge columns, centre: see kColumnOrder), so the candidate index is
-//   wave-uniform and its quantized coordinates come through the scalar cache.  Each lane
-//   applies the conservative integer filter (push_quant) and sets one bit of a 32-bit word per
-//   32 candidates.  Words never straddle two columns; `wj0[w]` is the sorted slot of the
-//   first candidate of word w.  Segments of up to 31 words hold whole column groups.
+// Phase 1 (test): the chunk's candidate words (32 consecutive slots of one column) are
+//   visited in the cursor's order (parts, passes, columns: see kPasses), and each lane gets
+//   one bit per candidate from the conservative integer filter: on the matrix cores
+//   (mfma_filter_word, 32 candidates x 64 targets per word) where the chunk's coordinates fit
+//   its exact range, else one candidate at a time for the whole wave (push_quant, candidate
+//   data through the scalar cache).  `wj0[w]` is the sorted slot of the first candidate of
+//   word w.  Segments hold up to kSeg - 1 words and never span two parts.
 // Phase 2 (evaluate): every lane walks only its own set bits and evaluates the kernel for
 //   those pairs (pairs the filter let through beyond the radius weigh exactly zero).
-// The density pass stores the words of each chunk in HBM (masks, wj0, word count, group
+// The density pass stores the words of each chunk in HBM (masks, wj0, word count, part
 // starts) and the force rows of each particle; the force pass reads them back instead of
 // testing again.  Chunks with more than kMaxWords words (pile-ups of thousands of particles
 // per cell) are re-tested by the force pass.  The neighbour-list export walks the reference's
@@ -167,8 +168,8 @@ __global__ __launch_bounds__(Shape::threads) void k_density_walk(
     Cursor cu;
     cursor_begin(cu, ch, cs, c, 0, part);
     if (lane == 0 && part <= 0) {
-        ggrp[item] = pack_groups(cu);
-        gnw[item] = (uint32_t)(cu.gw0 + cu.gw1 + cu.gw2 + cu.gw3);
+        ggrp[item] = pack_parts(cu);
+        gnw[item] = (uint32_t)(cu.pw0 + cu.pw1 + cu.pw2);
     }
     // per-part running sum `cur`, parts folded into `total` as (p0 + p1) + p2
     float total = 0.0f, cur = 0.0f;
@@ -312,8 +313,11 @@ struct ForceSums {
         const bool inner = q <= 1.0f;
         const float dwdq = inner ? q * __builtin_fmaf(1.5f, q, -2.0f) : -0.5f * t2 * t2;
         const float lapf = inner ? __builtin_fmaf(3.0f, q, -2.0f) : t2;
-        // gradW is zero below |r| = 1e-6 (src/kernels.cpp:96-98)
-        const float coef = rl >= 1e-6f ? __builtin_fmaf(p_half, a.w, b.w) * dwdq * rinv : 0.0f;
+        // gradW is zero below |r| = 1e-6 (src/kernels.cpp:96-98); a factor of 0 or 1 instead of
+        // a select keeps the step free of branches (the term is finite for every pair, so the
+        // product is +-0 where the reference's term is 0)
+        const float gate = rl >= 1e-6f ? 1.0f : 0.0f;
+        const float coef = __builtin_fmaf(p_half, a.w, b.w) * dwdq * (rinv * gate);
         fp.x -= coef * rx;
         fp.y -= coef * ry;
         fp.z -= coef * rz;
@@ -462,7 +466,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
             if (seg_base >= words_end) break;
             seg_part = part_of_word(seg_base, groups);
             const int pend = part_word_end(groups, seg_part, total_words);
-            nw = segment_end(seg_base, pend, groups, kSeg - 1) - seg_base;
+            nw = min(kSeg - 1, pend - seg_base);
             const size_t base = (size_t)item * kMaxWords + seg_base;
             sl.nonempty = 0;
             sl.count = 0;

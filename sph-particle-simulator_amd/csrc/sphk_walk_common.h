@@ -157,54 +157,45 @@ struct Chunk {
 
 // Candidate columns in walk order, as (dx + 2) * 5 + (dy + 2).  Row 0, balanced order: the
 // four corner columns, the centre, the four edge columns, then the ring two columns out
-// (walked only when axis_edge_flags asks for it).  A lane near one side of the chunk's
-// column finds about as many neighbours in each of the first three groups as a lane near
-// the other side, so segments that hold whole groups have even per-lane bit counts (the
-// wavefront runs the largest).  Row 1: the reference's (dx, dy) lexicographic order
-// (SpatialHash::query_squared), for the neighbour-list export.
+// (walked only when axis_edge_flags asks for it).  Row 1: the reference's (dx, dy)
+// lexicographic order (SpatialHash::query_squared), for the neighbour-list export.
 __constant__ uint8_t kColumnOrder[2][25] = {
     {6, 18, 8, 16, 12, 7, 17, 11, 13, 0, 1, 2, 3, 4, 5, 9, 10, 14, 15, 19, 20, 21, 22, 23, 24},
     {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24}};
 
-// Balance group of position k of the balanced order, and the first position of group g.
-__device__ __forceinline__ int column_group(int k) { return k < 4 ? 0 : k < 5 ? 1 : k < 9 ? 2 : 3; }
-__device__ __forceinline__ int group_begin(int g) { return g == 0 ? 0 : g == 1 ? 4 : g == 2 ? 5 : 9; }
+// Parts of a walk (balanced order): the corner columns with the centre (positions 0-4), the
+// edge columns (5-8), the outer ring (9-24).  Sums over a chunk's neighbours are formed per
+// part, each in candidate order, and combined as (0 + part 0) + part 1) + part 2, so a walk
+// split over three wavefronts (one part each) gives the same bits as one wavefront walking
+// all three.  Part starts are segment breaks.
+__device__ __forceinline__ int part_begin(int p) { return p == 0 ? 0 : p == 1 ? 5 : 9; }
+__device__ __forceinline__ int part_end(int p) { return p == 0 ? 5 : p == 1 ? 9 : 25; }
+__device__ __forceinline__ int part_of_position(int k) { return k < 5 ? 0 : k < 9 ? 1 : 2; }
 
-// Parts of a walk: the corner columns with the centre, the edge columns, the outer ring.
-// Sums over a chunk's neighbours are formed per part, each part in candidate order, and
-// combined as (part 0 + part 1) + part 2, so a walk split over three wavefronts (one part
-// each) gives the same bits as one wavefront walking all three.  Parts are runs of whole
-// groups: part p starts at group kPartGroup1 (p = 1) or kPartGroup2 (p = 2).  Part breaks
-// are segment breaks of the single-wavefront walk too; these parts put them where whole
-// groups end anyway in dense blocks (about 8 words per column), so that walk keeps the
-// segments it would have without parts, while the split walk runs the corners with the
-// centre and the edges side by side (the ring is rarely walked).
-constexpr int kPartGroup1 = 2;
-constexpr int kPartGroup2 = 3;
-__device__ __forceinline__ int part_of_group(int g) {
-    return g < kPartGroup1 ? 0 : g < kPartGroup2 ? 1 : 2;
-}
-__device__ __forceinline__ int part_first_group(int p) {
-    return p == 0 ? 0 : p == 1 ? kPartGroup1 : kPartGroup2;
-}
-// First position of part p in the balanced order, and the end of it.
-__device__ __forceinline__ int part_begin(int p) { return group_begin(part_first_group(p)); }
-__device__ __forceinline__ int part_end(int p) { return p == 2 ? 25 : part_begin(p + 1); }
+// Candidate order.  A word is 32 consecutive sorted slots of one candidate column (the last
+// word of a column may be shorter), candidate k of a word in bit 31 - k.  In the balanced
+// order each part is walked in kPasses passes: pass s visits the part's columns in order and
+// takes words s, s + kPasses, s + 2 kPasses, ... of each.  A column's words run up in z, so
+// every segment mixes all columns and z ranges of the part, and the neighbour counts of the
+// 64 lanes in a segment come out even (the wavefront runs the largest): a replay of the 1M
+// dam break gives ~300 steps per chunk with 31-word segments against ~326 walking each
+// column's words in turn (254 for one continuous walk).  The reference order walks every
+// word of each column in turn.
+constexpr int kPasses = 4;
 
-// Cursor over the candidate columns of a chunk.
+// Cursor over the candidate words of a chunk.
 struct Cursor {
-    int k;            // position in the column order
-    int kend;         // end of the positions walked
+    int k;            // position of the current column in the order
+    int kend;         // end of the positions of the current part
+    int part;         // current part (0 in the reference order)
+    int last_part;    // last part walked
+    int pass;         // current pass
     int order;        // 0 balanced, 1 reference
-    int group;        // balance group of the current column (0 in the reference order)
-    bool fresh;       // the current column opens its group
-    uint32_t j, end;  // current slot and end of the current column range
+    bool fresh;       // the next word opens a part
+    uint32_t j, end;  // next word's first slot; end of the current column range
+    uint32_t step;    // slots from one word of a column to the next walked one
     bool done;
-    int gw0, gw1, gw2, gw3;  // words (32 candidates, never straddling columns) of each group
-    // words of the current group (separate fields and selects keep the cursor in registers)
-    __device__ __forceinline__ int group_words() const {
-        return group == 0 ? gw0 : group == 1 ? gw1 : group == 2 ? gw2 : gw3;
-    }
+    int pw0, pw1, pw2;  // words of each part
 };
 
 __device__ __forceinline__ void column_range(const Chunk& ch, int dx, int dy,
@@ -224,41 +215,57 @@ __device__ __forceinline__ bool column_at(const Chunk& ch, int order, int k, int
     return dx >= ch.dx0 && dx <= ch.dx1 && dy >= ch.dy0 && dy <= ch.dy1;
 }
 
-// Moves to the first walked column at or after position k (done when none is left).
+// Moves to the first column at or after position k of the current pass that has a word in
+// this pass, continuing with the next pass and the next part (done when none is left).
 __device__ __forceinline__ void cursor_seek(Cursor& cu, const Chunk& ch,
                                             const uint32_t* __restrict__ cs, const StepConsts& c,
                                             int k) {
-    for (; k < cu.kend; ++k) {
-        int dx, dy;
-        if (!column_at(ch, cu.order, k, dx, dy)) continue;
-        uint32_t j, end;
-        column_range(ch, dx, dy, cs, c, j, end);
-        // an empty column holds no words and opens no group: the next non-empty column of
-        // its group does (segments and parts break where words of a new group start)
-        if (j >= end) continue;
-        const int g = cu.order == 0 ? column_group(k) : 0;
-        cu.fresh = g != cu.group;
-        cu.k = k;
-        cu.group = g;
-        cu.j = j;
-        cu.end = end;
+    for (;;) {
+        for (; k < cu.kend; ++k) {
+            int dx, dy;
+            if (!column_at(ch, cu.order, k, dx, dy)) continue;
+            uint32_t a, b;
+            column_range(ch, dx, dy, cs, c, a, b);
+            const uint32_t j = a + 32u * (uint32_t)cu.pass;
+            if (j >= b) continue;
+            cu.k = k;
+            cu.j = j;
+            cu.end = b;
+            return;
+        }
+        if (cu.order == 0 && cu.pass + 1 < kPasses) {
+            ++cu.pass;
+            k = part_begin(cu.part);
+            continue;
+        }
+        if (cu.order == 0 && cu.part < cu.last_part) {
+            ++cu.part;
+            cu.pass = 0;
+            cu.kend = part_end(cu.part);
+            cu.fresh = true;
+            k = part_begin(cu.part);
+            continue;
+        }
+        cu.done = true;
         return;
     }
-    cu.k = cu.kend;
-    cu.done = true;
 }
 
 // order 0: balanced (the SPH walks); 1: the reference's (neighbour-list export).  A part
-// p >= 0 (balanced order) limits the walk to the columns of that part; the word counts of
-// all groups are counted either way.
+// p >= 0 (balanced order) limits the walk to that part; the word counts of all parts are
+// counted either way.
 __device__ __forceinline__ void cursor_begin(Cursor& cu, const Chunk& ch,
                                              const uint32_t* __restrict__ cs, const StepConsts& c,
                                              int order = 0, int part = -1) {
     cu.order = order;
-    cu.kend = part >= 0 ? part_end(part) : 25;
+    cu.part = part >= 0 ? part : 0;
+    cu.last_part = order != 0 ? 0 : part >= 0 ? part : 2;
+    cu.kend = order != 0 ? 25 : part_end(cu.part);
+    cu.pass = 0;
+    cu.step = order != 0 ? 32u : 32u * kPasses;
     cu.done = false;
-    cu.group = -1;
-    cu.gw0 = cu.gw1 = cu.gw2 = cu.gw3 = 0;
+    cu.fresh = true;
+    cu.pw0 = cu.pw1 = cu.pw2 = 0;
     if (order == 0) {
         for (int k = 0; k < 25; ++k) {
             int dx, dy;
@@ -266,23 +273,21 @@ __device__ __forceinline__ void cursor_begin(Cursor& cu, const Chunk& ch,
             uint32_t a, b;
             column_range(ch, dx, dy, cs, c, a, b);
             const int w = (int)((b - a + 31u) / 32u);
-            const int g = column_group(k);
-            cu.gw0 += g == 0 ? w : 0;
-            cu.gw1 += g == 1 ? w : 0;
-            cu.gw2 += g == 2 ? w : 0;
-            cu.gw3 += g == 3 ? w : 0;
+            const int p = part_of_position(k);
+            cu.pw0 += p == 0 ? w : 0;
+            cu.pw1 += p == 1 ? w : 0;
+            cu.pw2 += p == 2 ? w : 0;
         }
     }
-    cursor_seek(cu, ch, cs, c, part >= 0 ? part_begin(part) : 0);
+    cursor_seek(cu, ch, cs, c, order != 0 ? 0 : part_begin(cu.part));
 }
 
 // Words of the chunk before part p (where its words start in the stored order).
 __device__ __forceinline__ int part_word_base(const Cursor& cu, int p) {
-    const int g = part_first_group(p);
-    return (g > 0 ? cu.gw0 : 0) + (g > 1 ? cu.gw1 : 0) + (g > 2 ? cu.gw2 : 0);
+    return (p > 0 ? cu.pw0 : 0) + (p > 1 ? cu.pw1 : 0);
 }
 
-// Moves to the next column when the current one is exhausted; false when all are.
+// Moves on when the current column has no word left in this pass; false when all are done.
 __device__ __forceinline__ bool cursor_settle(Cursor& cu, const Chunk& ch,
                                               const uint32_t* __restrict__ cs,
                                               const StepConsts& c) {
@@ -290,43 +295,22 @@ __device__ __forceinline__ bool cursor_settle(Cursor& cu, const Chunk& ch,
     return !cu.done;
 }
 
-// Group starts of a chunk's stored words, packed 8 bits each: words before group 1, 2, 3.
-__device__ __forceinline__ uint32_t pack_groups(const Cursor& cu) {
-    const uint32_t b1 = (uint32_t)cu.gw0;
-    const uint32_t b2 = b1 + (uint32_t)cu.gw1;
-    const uint32_t b3 = b2 + (uint32_t)cu.gw2;
-    return b1 | b2 << 8 | b3 << 16;
-}
-
-// First stored word of group g (from the packed group starts).
-__device__ __forceinline__ int group_word_start(uint32_t groups, int g) {
-    return g == 0 ? 0 : (int)((groups >> (8 * (g - 1))) & 0xffu);
+// Part starts of a chunk's stored words, packed 8 bits each: words before part 1, part 2.
+__device__ __forceinline__ uint32_t pack_parts(const Cursor& cu) {
+    const uint32_t b1 = (uint32_t)cu.pw0;
+    const uint32_t b2 = b1 + (uint32_t)cu.pw1;
+    return b1 | b2 << 8;
 }
 // First stored word of part p, and the end of its words.
-__device__ __forceinline__ int part_word_start(uint32_t groups, int p) {
-    return group_word_start(groups, part_first_group(p));
+__device__ __forceinline__ int part_word_start(uint32_t parts, int p) {
+    return p == 0 ? 0 : (int)((parts >> (8 * (p - 1))) & 0xffu);
 }
-__device__ __forceinline__ int part_word_end(uint32_t groups, int p, int total) {
-    return p == 2 ? total : part_word_start(groups, p + 1);
+__device__ __forceinline__ int part_word_end(uint32_t parts, int p, int total) {
+    return p == 2 ? total : part_word_start(parts, p + 1);
 }
 // Part of the stored word at `pos`.
-__device__ __forceinline__ int part_of_word(int pos, uint32_t groups) {
-    return pos < part_word_start(groups, 1) ? 0 : pos < part_word_start(groups, 2) ? 1 : 2;
-}
-
-// End of the stored-word segment that starts at word `pos`: never past the end of its part
-// (`total` is that end); the last group start (or that end) within `cap` words, or pos + cap
-// when the group at pos is larger than that.
-__device__ __forceinline__ int segment_end(int pos, int total, uint32_t groups, int cap) {
-    const int lim = min(pos + cap, total);
-    if (lim == total) return total;
-    int end = pos + cap;
-#pragma unroll
-    for (int g = 2; g >= 0; --g) {
-        const int b = (int)((groups >> (8 * g)) & 0xffu);
-        if (b > pos && b <= lim) return b;
-    }
-    return end;
+__device__ __forceinline__ int part_of_word(int pos, uint32_t parts) {
+    return pos < part_word_start(parts, 1) ? 0 : pos < part_word_start(parts, 2) ? 1 : 2;
 }
 
 // Appends one test result to a 32-bit word: word = 2*word + (d2 <= r2).  One compare into
@@ -434,11 +418,12 @@ __device__ __forceinline__ void mfma_filter_begin(MfmaFilter& mf, const Chunk& c
 }
 
 // Verdict word of this lane's target over candidates j0 .. j0 + cnt - 1 (cnt in 1..32),
-// candidate k at bit 31 - k, like the VALU test.  The rows of the column's next word (which
-// ends at `end`) are requested before this word's arithmetic.
+// candidate k at bit 31 - k, like the VALU test.  The rows of the column's next walked word
+// (`step` slots on, if before `end`) are requested before this word's arithmetic.
 __device__ __forceinline__ uint32_t mfma_filter_word(MfmaFilter& mf,
                                                      const uint2* __restrict__ sq, uint32_t j0,
-                                                     uint32_t cnt, uint32_t end, uint32_t lane) {
+                                                     uint32_t cnt, uint32_t end, uint32_t step,
+                                                     uint32_t lane) {
     const uint2 qb = mf.pref_j == j0 ? mf.pref : sq[j0 + min(mf.crow, cnt - 1u)];
     const sphk_short2 b = __builtin_bit_cast(sphk_short2, qb.x) - __builtin_bit_cast(sphk_short2, mf.ox);
     const uint32_t bzu = qb.y - mf.oz;
@@ -459,7 +444,7 @@ __device__ __forceinline__ uint32_t mfma_filter_word(MfmaFilter& mf,
     // next word's rows, requested once this word's are used up (the empty asm pins the A
     // operand before the request, so this word does not wait for the next one's load)
     asm volatile("" ::"v"(a));
-    const uint32_t jn = j0 + cnt;
+    const uint32_t jn = j0 + step;
     if (jn < end) {
         mf.pref = sq[jn + min(mf.crow, min(32u, end - jn) - 1u)];
         mf.pref_j = jn;
@@ -495,12 +480,9 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
     sl.count = 0;
     part = 0;
     while (w < kSeg - 1 && cursor_settle(cu, ch, cs, c)) {
-        const int p = part_of_group(cu.group);
-        if (w == 0) part = p;
+        if (w == 0) part = cu.part;
         if (cu.fresh) {
-            // a new part, or a group that does not fit in what is left of the segment,
-            // starts the next one
-            if (w > 0 && (p != part || w + cu.group_words() > kSeg - 1)) break;
+            if (w > 0) break;  // a new part starts the next segment
             cu.fresh = false;
         }
         const uint32_t j0 = uni(cu.j);
@@ -508,7 +490,7 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
         uint32_t word = 0;
         const uint2* __restrict__ pb = sq + j0;  // uniform base: scalar loads
         if (kMfma && mf->on) {
-            word = mfma_filter_word(*mf, sq, j0, cnt, cu.end, lane);
+            word = mfma_filter_word(*mf, sq, j0, cnt, cu.end, cu.step, lane);
         } else if (cnt == 32u) {
 #pragma unroll
             for (int k = 0; k < 32; ++k) word = push_quant(word, qi, pb[k], c.qneg);
@@ -521,7 +503,7 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
         if (lane == 0) lwj0[w] = j0;
         sl.nonempty |= (word != 0u ? 1u : 0u) << w;
         sl.count += (uint32_t)__builtin_popcount(word);
-        cu.j = j0 + cnt;
+        cu.j = j0 + cu.step;
         ++w;
     }
     return w;
@@ -736,7 +718,7 @@ __device__ __forceinline__ bool build_list(ListBuilder& lb, bool stored, int tot
                 ++lb.n_ne;
                 lb.cnt += (uint32_t)__builtin_popcount(word);
             }
-            cu.j = j0 + cnt;
+            cu.j = j0 + cu.step;
             any = true;
         }
     }
