@@ -43,7 +43,7 @@ struct StepConsts {
 };
 
 // Neighbour row of the force walk, one per sorted slot: a = (x, y, z, m/rho),
-// b = (vx, vy, vz, m P / (2 rho^2)); both halves share one 32-byte block.
+// b = (vx, vy, vz, m P / (4 rho^2)); both halves share one 32-byte block.
 struct alignas(32) ForceRow {
     float4 a, b;
 };

@@ -121,8 +121,9 @@ __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
     return acc;
 }
 
-// Density, the force rows of the particle (B = m/rho and C = B P / 2, so that
-// m_j (P_i + P_j) / (2 rho_j) = (P_i / 2) B_j + C_j) and rho in svr.w.
+// Density, the force rows of the particle (B = m/rho and C = B P / 4, so that
+// m_j (P_i + P_j) / (2 rho_j) = 2 ((P_i / 4) B_j + C_j); the 2 rides on dW/dq) and rho in
+// svr.w.
 __device__ __forceinline__ void density_epilogue(float sum, const float4& pi, uint32_t s,
                                                  const StepConsts& c, float4* __restrict__ svr,
                                                  ForceRow* __restrict__ rows) {
@@ -133,7 +134,7 @@ __device__ __forceinline__ void density_epilogue(float sum, const float4& pi, ui
     const float B = pi.w / rho;
     const float P = c.k_gas * (rho - c.rho0);
     rows[s].a = make_float4(pi.x, pi.y, pi.z, B);
-    rows[s].b = make_float4(v.x, v.y, v.z, 0.5f * B * P);
+    rows[s].b = make_float4(v.x, v.y, v.z, 0.25f * B * P);
 }
 
 // Density pass over one chunk (kSplit false: all parts; true: the part of this wavefront,
@@ -302,7 +303,7 @@ __device__ __forceinline__ float3 final_accel(float3 fp, float3 fv, float m_i, f
 struct ForceSums {
     float3 fp = make_float3(0.f, 0.f, 0.f);
     float3 fv = make_float3(0.f, 0.f, 0.f);
-    __device__ __forceinline__ void add(const float4& pi, const float3& vi, float p_half,
+    __device__ __forceinline__ void add(const float4& pi, const float3& vi, float p_quarter,
                                         const float4& a, const float4& b, const StepConsts& c) {
         const float rx = pi.x - a.x, ry = pi.y - a.y, rz = pi.z - a.z;
         const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
@@ -311,13 +312,13 @@ struct ForceSums {
         const float q = rl * c.inv_h;
         const float t2 = fmaxf(__builtin_fmaf(-c.inv_h, rl, 2.0f), 0.0f);
         const bool inner = q <= 1.0f;
-        const float dwdq = inner ? q * __builtin_fmaf(1.5f, q, -2.0f) : -0.5f * t2 * t2;
+        // twice dW/dq: q (3q - 4) inside, -(2 - q)^2 outside; the rows' C and p_quarter carry
+        // the matching halves, so coef is the same product as with dW/dq, bit for bit
+        const float dwdq2 = inner ? q * __builtin_fmaf(3.0f, q, -4.0f) : -t2 * t2;
         const float lapf = inner ? __builtin_fmaf(3.0f, q, -2.0f) : t2;
-        // gradW is zero below |r| = 1e-6 (src/kernels.cpp:96-98); a factor of 0 or 1 instead of
-        // a select keeps the step free of branches (the term is finite for every pair, so the
-        // product is +-0 where the reference's term is 0)
-        const float gate = rl >= 1e-6f ? 1.0f : 0.0f;
-        const float coef = __builtin_fmaf(p_half, a.w, b.w) * dwdq * (rinv * gate);
+        // gradW is zero below |r| = 1e-6 (src/kernels.cpp:96-98)
+        const float dw = rl >= 1e-6f ? dwdq2 : 0.0f;
+        const float coef = __builtin_fmaf(p_quarter, a.w, b.w) * dw * rinv;
         fp.x -= coef * rx;
         fp.y -= coef * ry;
         fp.z -= coef * rz;
@@ -336,7 +337,7 @@ __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                            const uint32_t* __restrict__ lmask,
                                            const uint32_t* __restrict__ lwj0, uint32_t lane,
                                            const ForceRow* __restrict__ rows, const float4& pi,
-                                           const float3& vi, float p_half, const StepConsts& c,
+                                           const float3& vi, float p_quarter, const StepConsts& c,
                                            ForceSums& f) {
     if (steps == 0) return;
     static_assert(sizeof(ForceRow) == 32, "rows are 32-byte blocks");
@@ -348,18 +349,18 @@ __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
     for (; t + 3 <= steps; t += 3) {
         o = bw.template next<5>(lmask, lwj0, lane);
         const float4 a2 = row_at(rows, o).a, b2 = row_at(rows, o).b;
-        f.add(pi, vi, p_half, a0, b0, c);
+        f.add(pi, vi, p_quarter, a0, b0, c);
         o = bw.template next<5>(lmask, lwj0, lane);
         a0 = row_at(rows, o).a;
         b0 = row_at(rows, o).b;
-        f.add(pi, vi, p_half, a1, b1, c);
+        f.add(pi, vi, p_quarter, a1, b1, c);
         o = bw.template next<5>(lmask, lwj0, lane);
         a1 = row_at(rows, o).a;
         b1 = row_at(rows, o).b;
-        f.add(pi, vi, p_half, a2, b2, c);
+        f.add(pi, vi, p_quarter, a2, b2, c);
     }
-    if (t < steps) f.add(pi, vi, p_half, a0, b0, c);
-    if (t + 1 < steps) f.add(pi, vi, p_half, a1, b1, c);
+    if (t < steps) f.add(pi, vi, p_quarter, a0, b0, c);
+    if (t + 1 < steps) f.add(pi, vi, p_quarter, a1, b1, c);
 }
 
 // Folds the sums of finished parts into the running total, which starts at +0:
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     const float3 vi = make_float3(vr.x, vr.y, vr.z);
     const float rho_i = vr.w;
     const float P_i = c.k_gas * (rho_i - c.rho0);
-    const float p_half = 0.5f * P_i;
+    const float p_quarter = 0.25f * P_i;
     ForceSums f;
     int cur_part = 0;
     const int total_words = (int)uni(gnw[item]);
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
         wave_sync();
         BitWalk<kSeg> bw;
         bw.ne = sl.nonempty;
-        force_bits<kSeg>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi, p_half, c,
+        force_bits<kSeg>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi, p_quarter, c,
                          f);
         seg_base += nw;
         wave_sync();
