@@ -98,7 +98,7 @@ SurScales scales_of(const sphk_engine* e, const SurrogateState* s) {
     sc.inv_r = 1.0f / e->params.neighbor_search_radius;
     sc.inv_v = s->cfg.inv_v;
     sc.inv_h = 1.0f / e->params.smoothing_length;
-    sc.w_norm = 1.5f;  // 1 / w_shape(0) = 1 / (2/3)
+    sc.w_norm = 1.5f;  // 1 / W(0) without sigma = 1 / (2/3)
     sc.rho0 = e->params.rest_density;
     sc.rho_scale = s->cfg.rho_scale;
     sc.msg_scale = s->cfg.msg_scale;

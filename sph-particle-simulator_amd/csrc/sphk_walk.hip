@@ -88,13 +88,24 @@ uint32_t max_chunks(uint32_t n, const StepConsts& c) { return (n + 63u) / 64u + 
 // runs over the set bits, self included.  Stores the neighbour words of every chunk and the
 // force walk's neighbour rows of every particle.
 // ---------------------------------------------------------------------------
+// The cubic spline (CubicSplineKernel::W_3d, src/kernels.cpp:58-66) as one expression:
+// 6 W(q) / sigma = (2 - q)^3 - 4 (1 - q)^3 for q <= 1
+// and (2 - q)^3 for 1 < q <= 2, i.e. t^3 - 4 s^3 with t = max(2 - q, 0), s = max(1 - q, 0):
+// nine operations and no select (the two-branch form takes eleven).  The walk sums
+// m_j (6 W / sigma) and scales by sigma / 6 at the end.
+__device__ __forceinline__ float w6_shape(float r, float inv_h) {
+    const float t = fmaxf(__builtin_fmaf(-inv_h, r, 2.0f), 0.0f);
+    const float s = fmaxf(__builtin_fmaf(-inv_h, r, 1.0f), 0.0f);
+    return __builtin_fmaf(s * s, -4.0f * s, t * t * t);
+}
+
 template <bool kCount>
 __device__ __forceinline__ float density_pair(const float4& pi, const float4& q,
                                               const StepConsts& c, float acc, uint32_t& cnt) {
     const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
     if (kCount) cnt += d2 <= c.r2 ? 1u : 0u;  // the reference's test
     // pairs the filter let through beyond the radius have q >= 2 and weigh zero
-    return acc + q.w * w_shape(__builtin_amdgcn_sqrtf(d2) * c.inv_h);
+    return __builtin_fmaf(q.w, w6_shape(__builtin_amdgcn_sqrtf(d2), c.inv_h), acc);
 }
 
 // Neighbour positions are requested two steps ahead, in three rotating register sets.
@@ -127,7 +138,7 @@ __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
 __device__ __forceinline__ void density_epilogue(float sum, const float4& pi, uint32_t s,
                                                  const StepConsts& c, float4* __restrict__ svr,
                                                  ForceRow* __restrict__ rows) {
-    const float rho = c.sigma * sum;
+    const float rho = c.sigma * (sum * (1.0f / 6.0f));
     float4 v = svr[s];
     v.w = rho;
     svr[s] = v;

@@ -132,17 +132,6 @@ __device__ __forceinline__ uint32_t swizzled_block() {
 }
 __device__ __forceinline__ int unii(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Cubic-spline shape without sigma (CubicSplineKernel::W_3d, src/kernels.cpp:58-66):
-// 2/3 - q^2 + q^3/2 on [0,1], (2-q)^3/6 on (1,2], 0 beyond; both pieces are computed and
-// one is selected.
-__device__ __forceinline__ float w_shape(float q) {
-    const float q2 = q * q;
-    const float win = (2.0f / 3.0f - q2) + 0.5f * q2 * q;
-    const float t = fmaxf(2.0f - q, 0.0f);
-    const float wout = (1.0f / 6.0f) * t * t * t;
-    return q <= 1.0f ? win : wout;
-}
-
 // Wave-uniform description of one chunk and its candidate columns.
 struct Chunk {
     uint32_t s0, s_end;   // sorted slots [s0, s_end)
