@@ -149,11 +149,10 @@ __device__ __forceinline__ void density_epilogue(float sum, const float4& pi, ui
 }
 
 // Density pass over one chunk (kSplit false: all parts; true: the part of this wavefront,
-// three wavefronts per chunk, the sums finished by k_density_finish).
+// three wavefronts per chunk, the sums finished by k_density_finish).  Seven wavefronts per
+// SIMD: the 15-word shapes' LDS allows it, and the walk is VALU-bound (a few spills outside
+// the walk loops; measured faster at 1M than the 6 the default register allocation gives).
 template <class Shape, bool kSplit>
-// Seven wavefronts per SIMD: the 15-word shapes' LDS allows it, and the walk is VALU-bound
-// (a few spills outside the walk loops; measured faster at 1M than the 6 the default register
-// allocation gives).
 __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_density_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const uint2* __restrict__ sq, float4* __restrict__ svr,
