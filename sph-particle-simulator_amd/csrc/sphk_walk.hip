@@ -179,22 +179,28 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
     init_sentinel<kSeg>(lmask, lwj0, lane, c);
     MfmaFilter mf;
     mfma_filter_begin(mf, ch, sq, qi, lane, c.qneg);
-    Cursor cu;
-    cursor_begin(cu, ch, cs, c, 0, part);
+    WordTable wt;
+    const bool tab = word_table_build(wt, ch, cs, c, part, lane);
     if (lane == 0 && part <= 0) {
-        ggrp[item] = pack_parts(cu);
-        gnw[item] = (uint32_t)(cu.pw0 + cu.pw1 + cu.pw2);
+        ggrp[item] = (uint32_t)wt.ps1 | (uint32_t)wt.ps2 << 8;
+        gnw[item] = (uint32_t)wt.total;
     }
+    Cursor cu;
+    if (!tab) cursor_begin(cu, ch, cs, c, 0, part);
     // per-part running sum `cur`, parts folded into `total` as (p0 + p1) + p2
     float total = 0.0f, cur = 0.0f;
     int cur_part = 0;
     uint32_t cnt = 0;
-    int seg_base = kSplit ? part_word_base(cu, part) : 0;
+    int seg_base = wt.first;
+    int ti = 0;  // next table entry
     for (;;) {
         SegLane sl;
         int seg_part;
-        const int nw = phase1_segment<kSeg, true>(cu, ch, sq, cs, c, qi, active, lmask, lwj0,
-                                                  lane, sl, seg_part, &mf);
+        const int nw =
+            tab ? phase1_table<kSeg>(wt, ti, sq, c, qi, active, lmask, lwj0, lane, sl, seg_part,
+                                     mf)
+                : phase1_segment<kSeg, true>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane,
+                                             sl, seg_part, &mf);
         if (nw == 0) break;
         if (!kSplit) {
             for (; cur_part < seg_part; ++cur_part) {
@@ -213,7 +219,7 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
             cur = density_bits<false>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur, cnt);
         seg_base += nw;
         wave_sync();
-        if (cu.done) break;
+        if (tab ? ti >= wt.n : cu.done) break;
     }
     if (kSplit) {
         if (active) {

@@ -313,6 +313,107 @@ __device__ __forceinline__ uint32_t push_bit(uint32_t word, float d2, float r2) 
     return out;
 }
 
+// ---------------------------------------------------------------------------
+// Word table: the words of a chunk's walked range in walk order (balanced order, parts,
+// passes), built once per chunk so that phase 1 reads each word's (first slot, count) with
+// one lane read instead of stepping the cursor (whose column visits cost a scalar-load round
+// trip each, four per column with the passes).  Lane l holds entries l and 64 + l; an entry
+// is j0 | (cnt - 1) << 27 (slots stay below 2^27).  Ranges of more than kTableWords words use
+// the cursor; both give the same sequence.
+// ---------------------------------------------------------------------------
+constexpr int kTableWords = 128;
+
+struct WordTable {
+    uint32_t e0, e1;  // this lane's entries (l, 64 + l)
+    int n;            // uniform: words in the table
+    int first;        // uniform: stored-word index of entry 0
+    int ps1, ps2;     // uniform: stored-word index where parts 1 and 2 start
+    int total;        // uniform: words of the whole chunk
+};
+
+// Words of a column of nw words in pass s: s, s + kPasses, ...
+__device__ __forceinline__ int pass_words(int nw, int s) {
+    return nw > s ? (nw - s + kPasses - 1) / kPasses : 0;
+}
+
+// Counts the chunk's words; fills the table for the range of `part` (all parts if < 0) when
+// it holds at most kTableWords words.  Returns whether the table is filled.
+__device__ __forceinline__ bool word_table_build(WordTable& t, const Chunk& ch,
+                                                 const uint32_t* __restrict__ cs,
+                                                 const StepConsts& c, int part, uint32_t lane) {
+    int nw[25];
+    int pw[3] = {0, 0, 0};
+    int W[3][kPasses] = {};
+#pragma unroll
+    for (int k = 0; k < 25; ++k) {
+        int dx, dy;
+        nw[k] = 0;
+        if (column_at(ch, 0, k, dx, dy)) {
+            uint32_t a, b;
+            column_range(ch, dx, dy, cs, c, a, b);
+            nw[k] = (int)((b - a + 31u) / 32u);
+        }
+        const int p = part_of_position(k);
+        pw[p] += nw[k];
+#pragma unroll
+        for (int s = 0; s < kPasses; ++s) W[p][s] += pass_words(nw[k], s);
+    }
+    t.ps1 = pw[0];
+    t.ps2 = pw[0] + pw[1];
+    t.total = t.ps2 + pw[2];
+    t.first = part > 0 ? (part == 1 ? t.ps1 : t.ps2) : 0;
+    t.n = part >= 0 ? pw[part] : t.total;
+    t.e0 = t.e1 = 0u;
+    if (t.n > kTableWords) return false;
+    const int i0 = t.first + (int)lane, i1 = i0 + 64;  // stored-word indices of this lane's entries
+    int base = 0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        int pre[kPasses];
+        int pb[kPasses];  // stored-word index where pass s of part p starts
+#pragma unroll
+        for (int s = 0; s < kPasses; ++s) {
+            pre[s] = 0;
+            pb[s] = s == 0 ? base : pb[s - 1] + W[p][s - 1];
+        }
+        if (part < 0 || part == p) {
+#pragma unroll
+            for (int k = part_begin(p); k < part_end(p); ++k) {
+                if (nw[k] == 0) continue;
+                int dx, dy;
+                column_at(ch, 0, k, dx, dy);
+                uint32_t a, b;
+                column_range(ch, dx, dy, cs, c, a, b);
+#pragma unroll
+                for (int s = 0; s < kPasses; ++s) {
+                    const int m = pass_words(nw[k], s);
+                    const int lo = pb[s] + pre[s];
+                    // entry for stored-word index i: word s + kPasses (i - lo) of the column
+                    if (i0 >= lo && i0 < lo + m) {
+                        const uint32_t j0 = a + 32u * (uint32_t)(s + kPasses * (i0 - lo));
+                        t.e0 = j0 | (min(32u, b - j0) - 1u) << 27;
+                    }
+                    if (i1 >= lo && i1 < lo + m) {
+                        const uint32_t j0 = a + 32u * (uint32_t)(s + kPasses * (i1 - lo));
+                        t.e1 = j0 | (min(32u, b - j0) - 1u) << 27;
+                    }
+                    pre[s] += m;
+                }
+            }
+        }
+        base += pw[p];
+    }
+    return true;
+}
+
+// Entry i (uniform, < n) of the table: first slot and count.
+__device__ __forceinline__ void word_table_entry(const WordTable& t, int i, uint32_t& j0,
+                                                 uint32_t& cnt) {
+    const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)(i < 64 ? t.e0 : t.e1), i & 63);
+    j0 = e & ((1u << 27) - 1u);
+    cnt = (e >> 27) + 1u;
+}
+
 // Per-lane summary of one LDS segment: which words hold a neighbour (bit w) and how many
 // neighbours in total.
 struct SegLane {
@@ -407,11 +508,11 @@ __device__ __forceinline__ void mfma_filter_begin(MfmaFilter& mf, const Chunk& c
 }
 
 // Verdict word of this lane's target over candidates j0 .. j0 + cnt - 1 (cnt in 1..32),
-// candidate k at bit 31 - k, like the VALU test.  The rows of the column's next walked word
-// (`step` slots on, if before `end`) are requested before this word's arithmetic.
+// candidate k at bit 31 - k, like the VALU test.  The rows of the next word to be tested
+// (jn .. jn + cntn - 1; jn = ~0: none) are requested before this word's arithmetic.
 __device__ __forceinline__ uint32_t mfma_filter_word(MfmaFilter& mf,
                                                      const uint2* __restrict__ sq, uint32_t j0,
-                                                     uint32_t cnt, uint32_t end, uint32_t step,
+                                                     uint32_t cnt, uint32_t jn, uint32_t cntn,
                                                      uint32_t lane) {
     const uint2 qb = mf.pref_j == j0 ? mf.pref : sq[j0 + min(mf.crow, cnt - 1u)];
     const sphk_short2 b = __builtin_bit_cast(sphk_short2, qb.x) - __builtin_bit_cast(sphk_short2, mf.ox);
@@ -433,9 +534,8 @@ __device__ __forceinline__ uint32_t mfma_filter_word(MfmaFilter& mf,
     // next word's rows, requested once this word's are used up (the empty asm pins the A
     // operand before the request, so this word does not wait for the next one's load)
     asm volatile("" ::"v"(a));
-    const uint32_t jn = j0 + step;
-    if (jn < end) {
-        mf.pref = sq[jn + min(mf.crow, min(32u, end - jn) - 1u)];
+    if (jn != ~0u) {
+        mf.pref = sq[jn + min(mf.crow, cntn - 1u)];
         mf.pref_j = jn;
     }
     const sphk_f32x16 z16 = {};
@@ -479,7 +579,9 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
         uint32_t word = 0;
         const uint2* __restrict__ pb = sq + j0;  // uniform base: scalar loads
         if (kMfma && mf->on) {
-            word = mfma_filter_word(*mf, sq, j0, cnt, cu.end, cu.step, lane);
+            const uint32_t jn = j0 + cu.step;
+            word = mfma_filter_word(*mf, sq, j0, cnt, jn < cu.end ? jn : ~0u,
+                                    jn < cu.end ? min(32u, cu.end - jn) : 0u, lane);
         } else if (cnt == 32u) {
 #pragma unroll
             for (int k = 0; k < 32; ++k) word = push_quant(word, qi, pb[k], c.qneg);
@@ -493,6 +595,54 @@ __device__ __forceinline__ int phase1_segment(Cursor& cu, const Chunk& ch,
         sl.nonempty |= (word != 0u ? 1u : 0u) << w;
         sl.count += (uint32_t)__builtin_popcount(word);
         cu.j = j0 + cu.step;
+        ++w;
+    }
+    return w;
+}
+
+// Phase 1 over the word table: the next segment of up to kSeg - 1 words from local entry `i`
+// (advanced), never past a part start, tested on the matrix cores (or the VALU test where the
+// chunk is too wide for them).  Same words and order as phase1_segment.
+template <int kSeg>
+__device__ __forceinline__ int phase1_table(const WordTable& t, int& i,
+                                            const uint2* __restrict__ sq, const StepConsts& c,
+                                            uint2 qi, bool active, uint32_t* __restrict__ lmask,
+                                            uint32_t* __restrict__ lwj0, uint32_t lane,
+                                            SegLane& sl, int& part, MfmaFilter& mf) {
+    static_assert(kSeg <= 32, "one nonempty bit per word");
+    int w = 0;
+    sl.nonempty = 0;
+    sl.count = 0;
+    const int g0 = t.first + i;
+    part = g0 < t.ps1 ? 0 : g0 < t.ps2 ? 1 : 2;
+    while (w < kSeg - 1 && i < t.n) {
+        const int g = t.first + i;
+        if (w > 0 && (g == t.ps1 || g == t.ps2)) break;  // a new part starts the next segment
+        uint32_t j0, cnt;
+        word_table_entry(t, i, j0, cnt);
+        j0 = uni(j0);
+        cnt = uni(cnt);
+        uint32_t word = 0;
+        if (mf.on) {
+            uint32_t jn = ~0u, cntn = 0u;
+            if (i + 1 < t.n) word_table_entry(t, i + 1, jn, cntn);
+            word = mfma_filter_word(mf, sq, j0, cnt, uni(jn), uni(cntn), lane);
+        } else {
+            const uint2* __restrict__ pb = sq + j0;  // uniform base: scalar loads
+            if (cnt == 32u) {
+#pragma unroll
+                for (int k = 0; k < 32; ++k) word = push_quant(word, qi, pb[k], c.qneg);
+            } else {
+                for (uint32_t k = 0; k < cnt; ++k) word = push_quant(word, qi, pb[k], c.qneg);
+                word <<= (32u - cnt);
+            }
+        }
+        word = active ? word : 0u;
+        lmask[w * 64 + lane] = word;
+        if (lane == 0) lwj0[w] = j0;
+        sl.nonempty |= (word != 0u ? 1u : 0u) << w;
+        sl.count += (uint32_t)__builtin_popcount(word);
+        ++i;
         ++w;
     }
     return w;
