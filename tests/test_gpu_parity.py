@@ -457,6 +457,30 @@ def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes, split):
     assert err.max() <= ACC_TOL
 
 
+def test_pileup_split_and_single_walks_bitexact(sphk):
+    # Pile-up chunks have more than 128 candidate words: the single-wavefront density walk
+    # steps its cursor through them, the split walk enumerates each part's words in a word
+    # table, and the force walk re-tests them.  Every route walks the same words in the same
+    # order, so all shapes give the same bits.
+    rng = np.random.default_rng(11)
+    pos = np.concatenate([rng.uniform(0.1, 0.13, size=(3000, 3)),
+                          rng.uniform(0.3, 0.5, size=(2000, 3))]).astype(np.float32)
+    names = sphk.walk_shape_names()
+    outs = []
+    for d, f in ((0, 0), (names.index("seg16_waves4_split"), names.index("seg32_waves4_split")),
+                 (1, 0)):
+        p = sphk.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
+        e = sphk.Engine(len(pos), p)
+        e.set_walk_shapes(d, f)
+        e.upload(pos)
+        for _ in range(2):
+            e.step(1e-4)
+        outs.append((e.positions(), e.velocities(), e.densities()))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_graph_replay_matches_eager(sphk, scenes):
     # a captured step replayed many times equals the eager step bit for bit, with eager work
     # (downloads, restores) between the replays
