@@ -659,7 +659,7 @@ using Shape0 = WalkShape<16, 4>;
 using Shape1 = WalkShape<32, 4>;
 using Shape2 = WalkShape<32, 2>;
 using Shape3 = WalkShape<32, 1>;
-using Shape4 = WalkShape<8, 4>;
+using Shape4 = WalkShape<24, 4>;
 
 const char* walk_shape_name(int v) {
     switch (v) {
@@ -667,7 +667,7 @@ const char* walk_shape_name(int v) {
         case 1: return "seg32_waves4";
         case 2: return "seg32_waves2";
         case 3: return "seg32_waves1";
-        case 4: return "seg8_waves4";
+        case 4: return "seg24_waves4";
         case 5: return "seg16_waves4_split";
         case 6: return "seg32_waves4_split";
         default: return "?";
