@@ -400,6 +400,19 @@ __device__ __forceinline__ void fold_parts(float* __restrict__ t, uint32_t lane,
     f = ForceSums();
 }
 
+// The same fold with the running total in registers (shapes whose occupancy LDS limits).
+__device__ __forceinline__ void fold_parts_reg(float (&t)[6], ForceSums& f, int n) {
+    if (n <= 0) return;
+    const float v[6] = {f.fp.x, f.fp.y, f.fp.z, f.fv.x, f.fv.y, f.fv.z};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        float x = t[k] + v[k];
+        if (n >= 2) x = x + 0.0f;
+        t[k] = x;
+    }
+    f = ForceSums();
+}
+
 // Acceleration, leapfrog, clamp and the caller-order writes of one target.
 __device__ __forceinline__ void force_epilogue(const ForceSums& f, float4 pi, float3 vi,
                                                float rho_i, float P_i, uint32_t s,
@@ -424,9 +437,12 @@ __device__ __forceinline__ void force_epilogue(const ForceSums& f, float4 pi, fl
 }
 
 // Force pass over one chunk (kSplit false: all parts; true: the part of this wavefront,
-// three wavefronts per chunk, finished by k_force_finish).
+// three wavefronts per chunk, finished by k_force_finish).  Shapes of 24 words and more keep
+// the parts' running total in registers: their LDS, not their registers, limits occupancy,
+// and the 1.5 KB per wavefront it frees lets the 28-word shape run 5 wavefronts per SIMD.
 template <class Shape, bool kSplit>
-__global__ __launch_bounds__(Shape::threads) void k_force_walk(
+__global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(Shape::force_waves)))
+void k_force_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const uint2* __restrict__ sq,
     const float4* __restrict__ svr, const ForceRow* __restrict__ rows,
@@ -443,10 +459,13 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
     const uint32_t wave = uni(threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
-    __shared__ float ltot_all[kSplit ? 1 : Shape::waves][kSplit ? 1 : 6 * 64];
+    constexpr bool kRegTot = !kSplit && kSeg >= 24;
+    constexpr bool kLdsTot = !kSplit && !kRegTot;
+    __shared__ float ltot_all[kLdsTot ? Shape::waves : 1][kLdsTot ? 6 * 64 : 1];
+    float rtot[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     uint32_t* lmask = lmask_all[wave];
     uint32_t* lwj0 = lwj0_all[wave];
-    float* ltot = ltot_all[kSplit ? 0 : wave];
+    float* ltot = ltot_all[kLdsTot ? wave : 0];
     const uint32_t wid = swizzled_block() * Shape::waves + wave;
     const uint32_t item = kSplit ? wid / kParts : wid;
     const int part = kSplit ? (int)(wid % kParts) : -1;
@@ -457,7 +476,7 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
     if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
     const uint2 qi = active ? sq[s] : make_uint2(0u, 0u);
     init_sentinel<kSeg>(lmask, lwj0, lane, c);
-    if (!kSplit) {
+    if (kLdsTot) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) ltot[k * 64 + lane] = 0.0f;
     }
@@ -503,7 +522,10 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
             if (nw == 0) break;
         }
         if (!kSplit) {
-            fold_parts(ltot, lane, f, seg_part - cur_part);
+            if (kRegTot)
+                fold_parts_reg(rtot, f, seg_part - cur_part);
+            else
+                fold_parts(ltot, lane, f, seg_part - cur_part);
             cur_part = seg_part;
         }
         wave_sync();
@@ -523,11 +545,19 @@ __global__ __launch_bounds__(Shape::threads) void k_force_walk(
         }
         return;
     }
-    fold_parts(ltot, lane, f, kParts - cur_part);
+    if (kRegTot)
+        fold_parts_reg(rtot, f, kParts - cur_part);
+    else
+        fold_parts(ltot, lane, f, kParts - cur_part);
     if (!active) return;
     ForceSums tot;
-    tot.fp = make_float3(ltot[0 * 64 + lane], ltot[1 * 64 + lane], ltot[2 * 64 + lane]);
-    tot.fv = make_float3(ltot[3 * 64 + lane], ltot[4 * 64 + lane], ltot[5 * 64 + lane]);
+    if (kRegTot) {
+        tot.fp = make_float3(rtot[0], rtot[1], rtot[2]);
+        tot.fv = make_float3(rtot[3], rtot[4], rtot[5]);
+    } else {
+        tot.fp = make_float3(ltot[0 * 64 + lane], ltot[1 * 64 + lane], ltot[2 * 64 + lane]);
+        tot.fv = make_float3(ltot[3 * 64 + lane], ltot[4 * 64 + lane], ltot[5 * 64 + lane]);
+    }
     force_epilogue(tot, pi, vi, rho_i, P_i, s, scenes[ch.scene], c, sid, dt_dev, dt_arg, pm, vel,
                    acc_out, rho_out, prs_out);
 }
@@ -657,7 +687,7 @@ __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
 // ---------------------------------------------------------------------------
 using Shape0 = WalkShape<16, 4>;
 using Shape1 = WalkShape<32, 4>;
-using Shape2 = WalkShape<32, 2>;
+using Shape2 = WalkShape<28, 4>;
 using Shape3 = WalkShape<32, 1>;
 using Shape4 = WalkShape<24, 4>;
 
@@ -665,7 +695,7 @@ const char* walk_shape_name(int v) {
     switch (v) {
         case 0: return "seg16_waves4";
         case 1: return "seg32_waves4";
-        case 2: return "seg32_waves2";
+        case 2: return "seg28_waves4";
         case 3: return "seg32_waves1";
         case 4: return "seg24_waves4";
         case 5: return "seg16_waves4_split";

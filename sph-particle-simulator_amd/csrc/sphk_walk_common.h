@@ -947,6 +947,9 @@ struct WalkShape {
     static constexpr int seg = kSeg;
     static constexpr int waves = kWaves;
     static constexpr int threads = 64 * kWaves;
+    // occupancy target of the force walk: 5 wavefronts per SIMD where the LDS allows it
+    // (24- and 28-word segments without the LDS running total), else what the registers give
+    static constexpr int force_waves = (kSeg == 24 || kSeg == 28) ? 5 : 1;
 };
 
 }  // namespace
