@@ -196,6 +196,26 @@ def test_edge_pairs_in_all_directions_against_brute_force(sphk, oracle):
         assert k + 1 in idx[offs[k]:offs[k + 1]].tolist()
 
 
+def test_matrix_core_filter_keeps_every_neighbour(sphk, oracle):
+    # An unjittered lattice (spacing R/4, 64 particles per cell) puts thousands of pairs within
+    # a few ulps of the radius, and its chunks are dense enough for the density walk's
+    # matrix-core candidate filter.  The walk counts the neighbours that pass the reference's
+    # exact test among the filter's candidates: the count must equal the brute-force count,
+    # so the filter dropped none.
+    g = np.arange(24, dtype=np.float64) * 0.01
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = (np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + [0.13, 0.21, 0.05]).astype(np.float32)
+    p = sphk.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
+    e = sphk.Engine(len(pos), p)
+    e.upload(pos)
+    e.step(1e-5)
+    st = e.stats()
+    boffs, bidx = oracle.brute_neighbors(pos, np.float32(R2))
+    counts = np.diff(boffs)
+    assert st.total_neighbors == int(counts.sum())
+    assert st.max_neighbors == int(counts.max())
+
+
 def test_far_from_origin_against_brute_force(sphk, oracle):
     # Coordinates near 1000 (cell index ~25,000): the quantized filter keeps its coordinates
     # modulo 2^16 (x, y) and 2^32 (z), so neighbour differences must survive the wrap-around.
