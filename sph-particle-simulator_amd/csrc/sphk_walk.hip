@@ -509,11 +509,20 @@ void k_force_walk(
             const size_t base = (size_t)item * kMaxWords + seg_base;
             sl.nonempty = 0;
             sl.count = 0;
-            for (int w = 0; w < nw; ++w) {
-                const uint32_t word = gmask[(base + w) * 64 + lane];
-                lmask[w * 64 + lane] = word;
-                sl.nonempty |= (word != 0u ? 1u : 0u) << w;
-                sl.count += (uint32_t)__builtin_popcount(word);
+            // all of the segment's word loads are issued before the first is used (a loop
+            // over nw would wait for each load of its remainder in turn); slots past nw
+            // re-read word nw - 1 and are dropped
+            uint32_t wv[kSeg - 1];
+#pragma unroll
+            for (int w = 0; w < kSeg - 1; ++w)
+                wv[w] = gmask[(base + (size_t)min(w, nw - 1)) * 64 + lane];
+#pragma unroll
+            for (int w = 0; w < kSeg - 1; ++w) {
+                if (w < nw) {
+                    lmask[w * 64 + lane] = wv[w];
+                    sl.nonempty |= (wv[w] != 0u ? 1u : 0u) << w;
+                    sl.count += (uint32_t)__builtin_popcount(wv[w]);
+                }
             }
             for (int w = (int)lane; w < nw; w += 64) lwj0[w] = gwj0[base + w];
         } else {
