@@ -359,14 +359,14 @@ int ensure_grid(sphk_engine* e) {
     const uint64_t chunks = (e->cap + 63) / 64 + ncols;
     WalkBuffers& wb = e->walk;
     if (chunks > wb.chunk_cap) {
-        for (uint32_t* p : {wb.items, wb.mask, wb.wj0, wb.nwords, wb.groups})
+        for (uint32_t* p : {wb.items, wb.mask, wb.wj0, wb.nwords, wb.pstarts})
             if (p) HIP_TRY(hipFree(p));
         size_t want = std::max<size_t>(chunks, wb.chunk_cap * 5 / 4);
         HIP_TRY(dalloc(&wb.items, want));
         HIP_TRY(dalloc(&wb.mask, want * kMaxWords * 64));
         HIP_TRY(dalloc(&wb.wj0, want * kMaxWords));
         HIP_TRY(dalloc(&wb.nwords, want));
-        HIP_TRY(dalloc(&wb.groups, want));
+        HIP_TRY(dalloc(&wb.pstarts, want));
         wb.chunk_cap = want;
         ++e->buffer_gen;
     }
@@ -651,7 +651,7 @@ int sphk_destroy(sphk_engine* e) {
         if (p) (void)hipFree(p);
     for (void* p : {(void*)e->walk.items, (void*)e->walk.nitems, (void*)e->walk.colcnt,
                     (void*)e->walk.mask, (void*)e->walk.wj0, (void*)e->walk.nwords,
-                    (void*)e->walk.groups, (void*)e->walk.part_acc, (void*)e->walk.part_cnt,
+                    (void*)e->walk.pstarts, (void*)e->walk.part_acc, (void*)e->walk.part_cnt,
                     (void*)e->walk.part_force, (void*)e->scan_scratch, (void*)e->d_scenes})
         if (p) (void)hipFree(p);
     if (e->sur) surrogate_free(e->sur);

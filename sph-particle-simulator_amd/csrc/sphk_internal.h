@@ -100,7 +100,7 @@ struct WalkBuffers {
     uint32_t* mask = nullptr;     // [chunks][kMaxWords][64] neighbour bits
     uint32_t* wj0 = nullptr;      // [chunks][kMaxWords] first slot of each word
     uint32_t* nwords = nullptr;   // [chunks] words of each chunk
-    uint32_t* groups = nullptr;   // [chunks] first stored word of parts 1 and 2, 8 bits each
+    uint32_t* pstarts = nullptr;  // [chunks] first stored word of parts 1 and 2, 8 bits each
     size_t chunk_cap = 0;
     size_t col_cap = 0;
     // per-part sums of the split shapes, [3][part_cap] by sorted slot (allocated only

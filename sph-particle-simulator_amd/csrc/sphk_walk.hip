@@ -159,7 +159,7 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
     ForceRow* __restrict__ rows, const uint32_t* __restrict__ skey,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     uint32_t* __restrict__ gmask, uint32_t* __restrict__ gwj0, uint32_t* __restrict__ gnw,
-    uint32_t* __restrict__ ggrp, PartSums ps, unsigned long long* __restrict__ stats) {
+    uint32_t* __restrict__ gpart, PartSums ps, unsigned long long* __restrict__ stats) {
     constexpr int kSeg = Shape::seg;
     __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
     __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
     WordTable wt;
     const bool tab = word_table_build(wt, ch, cs, c, part, lane);
     if (lane == 0 && part <= 0) {
-        ggrp[item] = (uint32_t)wt.ps1 | (uint32_t)wt.ps2 << 8;
+        gpart[item] = (uint32_t)wt.ps1 | (uint32_t)wt.ps2 << 8;
         gnw[item] = (uint32_t)wt.total;
     }
     Cursor cu;
@@ -449,7 +449,7 @@ void k_force_walk(
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sid,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     const uint32_t* __restrict__ gmask, const uint32_t* __restrict__ gwj0,
-    const uint32_t* __restrict__ gnw, const uint32_t* __restrict__ ggrp, PartSums ps,
+    const uint32_t* __restrict__ gnw, const uint32_t* __restrict__ gpart, PartSums ps,
     const float* __restrict__ dt_dev, float dt_arg,
     float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc_out, float* __restrict__ rho_out,
@@ -489,22 +489,22 @@ void k_force_walk(
     int cur_part = 0;
     const int total_words = (int)uni(gnw[item]);
     const bool stored = total_words <= kMaxWords;
-    const uint32_t groups = stored ? uni(ggrp[item]) : 0u;
+    const uint32_t pstarts = stored ? uni(gpart[item]) : 0u;
     Cursor cu;
     if (!stored) cursor_begin(cu, ch, cs, c, 0, part);
     // stored words of this wavefront: all, or those of its part
     int seg_base = 0, words_end = total_words;
     if (kSplit) {
-        seg_base = part_word_start(groups, part);
-        words_end = part_word_end(groups, part, total_words);
+        seg_base = part_word_start(pstarts, part);
+        words_end = part_word_end(pstarts, part, total_words);
     }
     for (;;) {
         int nw, seg_part;
         SegLane sl;
         if (stored) {
             if (seg_base >= words_end) break;
-            seg_part = part_of_word(seg_base, groups);
-            const int pend = part_word_end(groups, seg_part, total_words);
+            seg_part = part_of_word(seg_base, pstarts);
+            const int pend = part_word_end(pstarts, seg_part, total_words);
             nw = min(kSeg - 1, pend - seg_base);
             const size_t base = (size_t)item * kMaxWords + seg_base;
             sl.nonempty = 0;
@@ -758,7 +758,7 @@ hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const u
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
                                                   spm, sq, svr, rows, skey, cell_start, c,
                                                   scenes, wb.mask, wb.wj0, wb.nwords,
-                                                  wb.groups, ps, nbr_stats));
+                                                  wb.pstarts, ps, nbr_stats));
     if (walk_shape_split(shape))
         hipLaunchKernelGGL(k_density_finish, dim3((n + 255) / 256), dim3(256), 0, st, spm, svr,
                            rows, cell_start, c, ps, nbr_stats);
@@ -783,7 +783,7 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
                                                   spm, sq, svr, rows, skey, sid,
                                                   cell_start, c,
                                                   scenes,
-                                                  wb.mask, wb.wj0, wb.nwords, wb.groups, ps,
+                                                  wb.mask, wb.wj0, wb.nwords, wb.pstarts, ps,
                                                   dt_dev, dt, pm, vel,
                                                   acc, rho, prs));
     if (walk_shape_split(shape))
