@@ -528,9 +528,8 @@ int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev) {
     HIP_TRY(launch_force_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->sid,
                               e->cell_start, n,
                               e->consts, e->d_scenes, adaptive ? e->fscal : nullptr, dt, e->pm,
-                              e->vel, e->acc, e->rho, e->prs, e->shape_force, st));
+                              e->vel, e->acc, e->rho, e->prs, e->shape_force, e->fscal + 1, st));
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
-    HIP_TRY(launch_advance_time(e->fscal + 1, adaptive ? e->fscal : nullptr, dt, st));
     return SPHK_OK;
 }
 

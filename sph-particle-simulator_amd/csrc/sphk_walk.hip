@@ -608,12 +608,15 @@ __global__ __launch_bounds__(256) void k_force_finish(
 
 // Particles with a non-finite coordinate (dense index G) have no neighbour, not even
 // themselves: rho = m W(0), a = (0, g, 0); they are integrated and clamped like the rest.
+// The step's last kernel: it also advances the simulation time (time_dev, if given) by dt.
 __global__ __launch_bounds__(256) void k_nonfinite_step(
     const float4* __restrict__ spm, float4* __restrict__ svr, const uint32_t* __restrict__ sid,
     const uint32_t* __restrict__ cs, uint32_t n, StepConsts c,
     const SceneGrid* __restrict__ scenes, const float* __restrict__ dt_dev,
     float dt_arg, float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc_out,
-    float* __restrict__ rho_out, float* __restrict__ prs_out) {
+    float* __restrict__ rho_out, float* __restrict__ prs_out, float* __restrict__ time_dev) {
+    if (time_dev && blockIdx.x == 0 && threadIdx.x == 0)
+        *time_dev = *time_dev + (dt_dev ? *dt_dev : dt_arg);
     const uint32_t first = cs[c.G];
     const uint32_t s = first + blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -772,8 +775,8 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
                              const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
                              float4* vel, float4* acc, float* rho, float* prs, int shape,
-                             hipStream_t st) {
-    if (n == 0) return hipSuccess;
+                             float* time_dev, hipStream_t st) {
+    if (n == 0) return time_dev ? launch_advance_time(time_dev, dt_dev, dt, st) : hipSuccess;
     PartSums ps;
     if (hipError_t err = part_sums(wb, n, shape, &ps)) return err;
     const uint32_t chunks = max_chunks(n, c);
@@ -792,7 +795,7 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
     // the non-finite bin holds at most n particles; the kernel reads its start on the device
     hipLaunchKernelGGL(k_nonfinite_step, dim3((n + 255) / 256), dim3(256), 0, st, spm,
                        const_cast<float4*>(svr), sid, cell_start, n, c, scenes, dt_dev, dt, pm, vel,
-                       acc, rho, prs);
+                       acc, rho, prs, time_dev);
     return hipGetLastError();
 }
 
