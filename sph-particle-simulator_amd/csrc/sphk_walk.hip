@@ -211,7 +211,7 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
         wave_sync();
         store_words(item, seg_base, nw, lmask, lwj0, gmask, gwj0, lane);
         BitWalk<kSeg> bw;
-        bw.ne = sl.nonempty;
+        bw.begin(sl.nonempty, lmask, lwj0, lane);
         const uint32_t steps = wave_max(sl.count);
         if (stats || kSplit)
             cur = density_bits<true>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur, cnt);
@@ -539,7 +539,7 @@ void k_force_walk(
         }
         wave_sync();
         BitWalk<kSeg> bw;
-        bw.ne = sl.nonempty;
+        bw.begin(sl.nonempty, lmask, lwj0, lane);
         force_bits<kSeg>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi, p_quarter, c,
                          f);
         seg_base += nw;
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
         if (nw == 0) break;
         wave_sync();
         BitWalk<kSeg> bw;
-        bw.ne = sl.nonempty;
+        bw.begin(sl.nonempty, lmask, lwj0, lane);
         for (uint32_t t = 0; t < sl.count; ++t) {
             const uint32_t j = bw.next(lmask, lwj0, lane);
             const float4 q = spm[j];

@@ -703,18 +703,32 @@ __device__ __forceinline__ bool load_chunk(uint32_t item, const uint32_t* __rest
 template <int kSeg>
 struct BitWalk {
     uint32_t m = 0, jb = 0, ne = 0;
+    uint32_t mq = 0, jq = 0;  // the next non-empty word (or the sentinel), read ahead
+    // starts the walk of a segment whose non-empty words are `nonempty`; the first word is
+    // read here, and each refill reads the one after, so a refill never waits for LDS
+    __device__ __forceinline__ void begin(uint32_t nonempty, const uint32_t* __restrict__ lmask,
+                                          const uint32_t* __restrict__ lwj0, uint32_t lane) {
+        ne = nonempty;
+        m = 0u;
+        fetch(lmask, lwj0, lane);
+    }
+    __device__ __forceinline__ void fetch(const uint32_t* __restrict__ lmask,
+                                          const uint32_t* __restrict__ lwj0, uint32_t lane) {
+        const uint32_t w = (uint32_t)__builtin_ctz(ne | (1u << (kSeg - 1)));
+        mq = lmask[w * 64 + lane];
+        jq = lwj0[w];
+        ne &= ne - 1u;
+    }
     // next candidate: its slot (kShift 0) or its byte offset in rows of 2^kShift bytes, so
     // a gather needs one 32-bit add-shift and a load with the row base in SGPRs
     template <int kShift = 0>
     __device__ __forceinline__ uint32_t next(const uint32_t* __restrict__ lmask,
                                              const uint32_t* __restrict__ lwj0, uint32_t lane) {
-        const uint32_t w = (uint32_t)__builtin_ctz(ne | (1u << (kSeg - 1)));
-        const uint32_t mn = lmask[w * 64 + lane];
-        const uint32_t jn = lwj0[w];
-        const bool need = m == 0u;
-        m = need ? mn : m;
-        jb = need ? (jn << kShift) : jb;
-        ne = need ? (ne & (ne - 1u)) : ne;
+        if (m == 0u) {
+            m = mq;
+            jb = jq << kShift;
+            fetch(lmask, lwj0, lane);
+        }
         const uint32_t k = (uint32_t)__builtin_clz(m);
         m ^= 0x80000000u >> k;
         return jb + (k << kShift);
