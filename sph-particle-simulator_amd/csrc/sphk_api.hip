@@ -83,9 +83,11 @@ StepConsts make_consts(const sphk_params& p) {
     float q0 = 0.0f;
     float w0 = (2.0f / 3.0f - q0 * q0) + ((0.5f * q0) * q0) * q0;
     c.w0 = c.sigma * w0;
-    c.grad_scale = c.sigma / h;
+    // the force walk sums dW/dq / 4 and half the Laplacian term; the factors 4 and 2 scale
+    // exactly
+    c.grad_scale = 4.0f * (c.sigma / h);
     float h_sq = h * h;
-    c.lap_scale = (c.sigma * p.viscosity) / h_sq;
+    c.lap_scale = 2.0f * ((c.sigma * p.viscosity) / h_sq);
     c.k_gas = p.gas_constant;
     c.rho0 = p.rest_density;
     c.grav = p.gravity;

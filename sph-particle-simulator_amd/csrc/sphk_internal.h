@@ -28,8 +28,8 @@ struct StepConsts {
     float h, inv_h;
     float sigma;     // 1/(pi h^3) (x 3/2 for the consistent formulation)
     float w0;        // W(0) = sigma * (2/3)
-    float grad_scale;  // sigma / h          (applied once per particle)
-    float lap_scale;   // sigma * mu / h^2   (applied once per particle)
+    float grad_scale;  // 4 sigma / h          (applied once per particle)
+    float lap_scale;   // 2 sigma * mu / h^2   (applied once per particle)
     float k_gas, rho0, grav, damping;
     int32_t formulation;
     uint32_t G;       // cells of all scene grids; particles with dense index G have a

@@ -90,22 +90,28 @@ uint32_t max_chunks(uint32_t n, const StepConsts& c) { return (n + 63u) / 64u + 
 // ---------------------------------------------------------------------------
 // The cubic spline (CubicSplineKernel::W_3d, src/kernels.cpp:58-66) as one expression:
 // 6 W(q) / sigma = (2 - q)^3 - 4 (1 - q)^3 for q <= 1
-// and (2 - q)^3 for 1 < q <= 2, i.e. t^3 - 4 s^3 with t = max(2 - q, 0), s = max(1 - q, 0):
-// nine operations and no select (the two-branch form takes eleven).  The walk sums
-// m_j (6 W / sigma) and scales by sigma / 6 at the end.
-__device__ __forceinline__ float w6_shape(float r, float inv_h) {
-    const float t = fmaxf(__builtin_fmaf(-inv_h, r, 2.0f), 0.0f);
-    const float s = fmaxf(__builtin_fmaf(-inv_h, r, 1.0f), 0.0f);
-    return __builtin_fmaf(s * s, -4.0f * s, t * t * t);
+// and (2 - q)^3 for 1 < q <= 2, i.e. t^3 - 4 s^3 with t = max(2 - q, 0), s = max(1 - q, 0).
+// The walk sums an eighth of it, u^3 + s^2 (-s / 2) with u = t / 2 = max(1 - q / 2, 0): every
+// rounding is that of the t-form scaled by 2^-3 (terms below 2^-126 aside), so the sums are
+// the same bits / 8; u and s lie in [0, 1], where the max is the instructions' clamp bit
+// (seven operations, no select).  The sum is scaled by 8 sigma / 6 at the end.
+__device__ __forceinline__ float w6_shape8(float r, float half_inv_h, float inv_h) {
+    const float u = __builtin_amdgcn_fmed3f(__builtin_fmaf(-half_inv_h, r, 1.0f), 0.0f, 1.0f);
+    const float s = __builtin_amdgcn_fmed3f(__builtin_fmaf(-inv_h, r, 1.0f), 0.0f, 1.0f);
+    return __builtin_fmaf(s * s, -0.5f * s, u * u * u);
 }
 
+// One neighbour's m_j (6 W / 8 sigma); |r| from the fused d^2 (one rounding fewer than the
+// reference's test, which the neighbour count keeps).
 template <bool kCount>
 __device__ __forceinline__ float density_pair(const float4& pi, const float4& q,
-                                              const StepConsts& c, float acc, uint32_t& cnt) {
-    const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
-    if (kCount) cnt += d2 <= c.r2 ? 1u : 0u;  // the reference's test
+                                              const StepConsts& c, float half_inv_h, float acc,
+                                              uint32_t& cnt) {
+    const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+    if (kCount) cnt += dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z) <= c.r2 ? 1u : 0u;
+    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     // pairs the filter let through beyond the radius have q >= 2 and weigh zero
-    return __builtin_fmaf(q.w, w6_shape(__builtin_amdgcn_sqrtf(d2), c.inv_h), acc);
+    return __builtin_fmaf(q.w, w6_shape8(__builtin_amdgcn_sqrtf(d2), half_inv_h, c.inv_h), acc);
 }
 
 // Neighbour positions are requested two steps ahead, in three rotating register sets.
@@ -116,19 +122,20 @@ __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                               const float4* __restrict__ spm, const float4& pi,
                                               const StepConsts& c, float acc, uint32_t& cnt) {
     if (steps == 0) return acc;
+    const float hh = 0.5f * c.inv_h;
     float4 q0 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
     float4 q1 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
     uint32_t t = 0;
     for (; t + 3 <= steps; t += 3) {
         const float4 q2 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
-        acc = density_pair<kCount>(pi, q0, c, acc, cnt);
+        acc = density_pair<kCount>(pi, q0, c, hh, acc, cnt);
         q0 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
-        acc = density_pair<kCount>(pi, q1, c, acc, cnt);
+        acc = density_pair<kCount>(pi, q1, c, hh, acc, cnt);
         q1 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
-        acc = density_pair<kCount>(pi, q2, c, acc, cnt);
+        acc = density_pair<kCount>(pi, q2, c, hh, acc, cnt);
     }
-    if (t < steps) acc = density_pair<kCount>(pi, q0, c, acc, cnt);
-    if (t + 1 < steps) acc = density_pair<kCount>(pi, q1, c, acc, cnt);
+    if (t < steps) acc = density_pair<kCount>(pi, q0, c, hh, acc, cnt);
+    if (t + 1 < steps) acc = density_pair<kCount>(pi, q1, c, hh, acc, cnt);
     return acc;
 }
 
@@ -138,7 +145,7 @@ __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
 __device__ __forceinline__ void density_epilogue(float sum, const float4& pi, uint32_t s,
                                                  const StepConsts& c, float4* __restrict__ svr,
                                                  ForceRow* __restrict__ rows) {
-    const float rho = c.sigma * (sum * (1.0f / 6.0f));
+    const float rho = c.sigma * (sum * (8.0f / 6.0f));  // 8.0f / 6.0f == 8 * (1.0f / 6.0f)
     float4 v = svr[s];
     v.w = rho;
     svr[s] = v;
@@ -328,20 +335,22 @@ struct ForceSums {
         const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
         const float rinv = __builtin_amdgcn_rsqf(fmaxf(d2, 1e-30f));
         const float rl = d2 * rinv;
-        const float q = rl * c.inv_h;
-        const float t2 = fmaxf(__builtin_fmaf(-c.inv_h, rl, 2.0f), 0.0f);
-        const bool inner = q <= 1.0f;
-        // twice dW/dq: q (3q - 4) inside, -(2 - q)^2 outside; the rows' C and p_quarter carry
-        // the matching halves, so coef is the same product as with dW/dq, bit for bit
-        const float dwdq2 = inner ? q * __builtin_fmaf(3.0f, q, -4.0f) : -t2 * t2;
-        const float lapf = inner ? __builtin_fmaf(3.0f, q, -2.0f) : t2;
+        // u = (2 - q) / 2 and s = 1 - q, clamped to [0, 1] by the instructions' clamp bit
+        // (seven operations, no select): dwq = (s - u)(s + u) = s^2 - u^2 is q (3q - 4) / 4
+        // for q <= 1 and -(2 - q)^2 / 4 beyond, proportional to dW/dq; lap = u - 2 s is
+        // (3q - 2) / 2 and (2 - q) / 2.  grad_scale and lap_scale carry the constant factors.
+        const float u =
+            __builtin_amdgcn_fmed3f(__builtin_fmaf(-0.5f * c.inv_h, rl, 1.0f), 0.0f, 1.0f);
+        const float s = __builtin_amdgcn_fmed3f(__builtin_fmaf(-c.inv_h, rl, 1.0f), 0.0f, 1.0f);
+        const float dwq = (s - u) * (s + u);
+        const float lap = __builtin_fmaf(-2.0f, s, u);
         // gradW is zero below |r| = 1e-6 (src/kernels.cpp:96-98)
-        const float dw = rl >= 1e-6f ? dwdq2 : 0.0f;
+        const float dw = rl >= 1e-6f ? dwq : 0.0f;
         const float coef = __builtin_fmaf(p_quarter, a.w, b.w) * dw * rinv;
         fp.x -= coef * rx;
         fp.y -= coef * ry;
         fp.z -= coef * rz;
-        const float cv = a.w * lapf;
+        const float cv = a.w * lap;
         fv.x += cv * (b.x - vi.x);
         fv.y += cv * (b.y - vi.y);
         fv.z += cv * (b.z - vi.z);
@@ -511,11 +520,13 @@ void k_force_walk(
             sl.count = 0;
             // all of the segment's word loads are issued before the first is used (a loop
             // over nw would wait for each load of its remainder in turn); slots past nw
-            // re-read word nw - 1 and are dropped
+            // re-read word nw - 1 and are dropped.  The words are read once: nontemporal
+            // loads keep them from evicting the neighbour rows from L2
             uint32_t wv[kSeg - 1];
 #pragma unroll
             for (int w = 0; w < kSeg - 1; ++w)
-                wv[w] = gmask[(base + (size_t)min(w, nw - 1)) * 64 + lane];
+                wv[w] = __builtin_nontemporal_load(
+                    &gmask[(base + (size_t)min(w, nw - 1)) * 64 + lane]);
 #pragma unroll
             for (int w = 0; w < kSeg - 1; ++w) {
                 if (w < nw) {

@@ -766,7 +766,8 @@ __device__ __forceinline__ void add_nbr_stats(uint32_t cnt, unsigned long long* 
     }
 }
 
-// Stores the LDS words [0, nw) of segment `seg_base` of this chunk into HBM.
+// Stores the LDS words [0, nw) of segment `seg_base` of this chunk into HBM (nontemporal: the
+// force walk reads them once, long after, and the rows the walks gather should keep L2).
 __device__ __forceinline__ void store_words(uint32_t item, int seg_base, int nw,
                                             const uint32_t* __restrict__ lmask,
                                             const uint32_t* __restrict__ lwj0,
@@ -776,7 +777,7 @@ __device__ __forceinline__ void store_words(uint32_t item, int seg_base, int nw,
     for (int w = 0; w < nw; ++w) {
         int gw = seg_base + w;
         if (gw >= kMaxWords) break;
-        gmask[(base + gw) * 64 + lane] = lmask[w * 64 + lane];
+        __builtin_nontemporal_store(lmask[w * 64 + lane], &gmask[(base + gw) * 64 + lane]);
     }
     for (int w = (int)lane; w < nw; w += 64) {
         int gw = seg_base + w;
