@@ -101,15 +101,14 @@ __device__ __forceinline__ float w6_shape8(float r, float half_inv_h, float inv_
     return __builtin_fmaf(s * s, -0.5f * s, u * u * u);
 }
 
-// One neighbour's m_j (6 W / 8 sigma); |r| from the fused d^2 (one rounding fewer than the
-// reference's test, which the neighbour count keeps).
+// One neighbour's m_j (6 W / 8 sigma).  |r| comes from the reference's d^2, which the
+// neighbour count (on every step of the engine) needs anyway.
 template <bool kCount>
 __device__ __forceinline__ float density_pair(const float4& pi, const float4& q,
                                               const StepConsts& c, float half_inv_h, float acc,
                                               uint32_t& cnt) {
-    const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
-    if (kCount) cnt += dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z) <= c.r2 ? 1u : 0u;
-    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
+    if (kCount) cnt += d2 <= c.r2 ? 1u : 0u;  // the reference's test
     // pairs the filter let through beyond the radius have q >= 2 and weigh zero
     return __builtin_fmaf(q.w, w6_shape8(__builtin_amdgcn_sqrtf(d2), half_inv_h, c.inv_h), acc);
 }
