@@ -516,15 +516,17 @@ __device__ __forceinline__ uint32_t mfma_filter_word(MfmaFilter& mf,
                                                      uint32_t lane) {
     const uint2 qb = mf.pref_j == j0 ? mf.pref : sq[j0 + min(mf.crow, cnt - 1u)];
     const sphk_short2 b = __builtin_bit_cast(sphk_short2, qb.x) - __builtin_bit_cast(sphk_short2, mf.ox);
-    const uint32_t bzu = qb.y - mf.oz;
-    const int32_t bz = (int32_t)bzu;
-    const int32_t bb = __builtin_amdgcn_sdot2(b, b, (int32_t)(bzu * bzu), false);
+    // candidate offsets from the origin are below 2^11 per axis (the candidates lie within
+    // the chunk's two-cell reach), so bz^2 is a 24-bit product and every A entry converts
+    // from 16 bits in one instruction
+    const int16_t bz = (int16_t)(qb.y - mf.oz);
+    const int32_t bb = __builtin_amdgcn_sdot2(b, b, (int32_t)bz * (int32_t)bz, false);
     sphk_half8 a;
-    a[0] = (_Float16)(int32_t)b.x;
-    a[1] = (_Float16)(int32_t)b.y;
+    a[0] = (_Float16)b.x;
+    a[1] = (_Float16)b.y;
     a[2] = (_Float16)bz;
-    a[3] = (_Float16)(bb & 2047);
-    a[4] = (_Float16)(bb >> 11);
+    a[3] = (_Float16)(int16_t)(bb & 2047);
+    a[4] = (_Float16)(int16_t)(bb >> 11);
     a[5] = (_Float16)1.0f;
     a[6] = (_Float16)2048.0f;
     a[7] = (_Float16)0.0f;
