@@ -708,18 +708,19 @@ struct BitWalk {
     uint32_t mq = 0, jq = 0;  // the next non-empty word (or the sentinel), read ahead
     // starts the walk of a segment whose non-empty words are `nonempty`; the first word is
     // read here, and each refill reads the one after, so a refill never waits for LDS
+    // `ne` always holds the sentinel word's bit, the highest, which is never cleared
     __device__ __forceinline__ void begin(uint32_t nonempty, const uint32_t* __restrict__ lmask,
                                           const uint32_t* __restrict__ lwj0, uint32_t lane) {
-        ne = nonempty;
+        ne = nonempty | (1u << (kSeg - 1));
         m = 0u;
         fetch(lmask, lwj0, lane);
     }
     __device__ __forceinline__ void fetch(const uint32_t* __restrict__ lmask,
                                           const uint32_t* __restrict__ lwj0, uint32_t lane) {
-        const uint32_t w = (uint32_t)__builtin_ctz(ne | (1u << (kSeg - 1)));
+        const uint32_t w = (uint32_t)__builtin_ctz(ne);
         mq = lmask[w * 64 + lane];
         jq = lwj0[w];
-        ne &= ne - 1u;
+        ne = (ne & (ne - 1u)) | (1u << (kSeg - 1));
     }
     // next candidate: its slot (kShift 0) or its byte offset in rows of 2^kShift bytes, so
     // a gather needs one 32-bit add-shift and a load with the row base in SGPRs
