@@ -33,7 +33,7 @@ ALG_BYTES = {"density": 16.0, "force": 52.0, "step": 149.0}
 # FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB device copy (fetch x2.0, write x1.0); see
 # profiles/force_walk_dam_break_1M_traffic.md.  The stored neighbour words it reads back and the
 # neighbour rows that miss the L2.
-FORCE_TRAFFIC_BYTES_PER_PARTICLE = 531.0
+FORCE_TRAFFIC_BYTES_PER_PARTICLE = 530.0
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # published; counts packed FP32 (two lanes per VALU slot)
 FP32_PEAK_UNPACKED_TFLOPS = 78.6  # one wave64 FP32 instruction per SIMD per 4 cycles
