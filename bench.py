@@ -29,11 +29,11 @@ METRIC = "particle-updates/s @1M dam break: plain GPU SPH vs surrogate at ≤1% 
 #   force pass:    read x,y,z,v,rho 28 + write x,v 24                = 52
 #   whole step:    keys 16 + histogram 4 + scan ~0.7 + reorder 60 + 16 + 52 ~= 149
 ALG_BYTES = {"density": 16.0, "force": 52.0, "step": 149.0}
-# HBM bytes the force walk moves per particle on the 1M dam break (seg32_waves4), from rocprofv3
-# FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB device copy (fetch x2.0, write x1.0); see
-# profiles/force_walk_dam_break_1M_traffic.md.  The stored neighbour words it reads back and the
-# neighbour rows that miss the L2.
-FORCE_TRAFFIC_BYTES_PER_PARTICLE = 530.0
+# Measured HBM traffic and VALU issue of the force walk on the 1M dam break: rocprofv3 counter
+# runs of the build named in the file (FETCH_SIZE and WRITE_SIZE, each scaled by what the same
+# counter reads for a device copy of known size).  Reported only for the launch shape the
+# counters were taken with.
+COUNTER_PROFILE = os.path.join(REPO, "profiles", "force_walk_dam_break_1M_counters.json")
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # published; counts packed FP32 (two lanes per VALU slot)
 FP32_PEAK_UNPACKED_TFLOPS = 78.6  # one wave64 FP32 instruction per SIMD per 4 cycles
@@ -56,8 +56,11 @@ def parse():
                     help="skip the batched-ensemble and graph-mode lines")
     ap.add_argument("--ensemble-scale", type=float, default=1.0,
                     help="particle-count scale of the 64-scene ensemble (1.0 = full size)")
-    ap.add_argument("--continuous", type=int, default=0,
-                    help="also time N continuous steps of the consistent formulation at dt=1e-4")
+    ap.add_argument("--full-run", type=int, default=1000,
+                    help="consecutive steps of the full run (consistent formulation, dt=1e-4) "
+                         "with its energy trajectory; 0 skips it")
+    ap.add_argument("--no-cpu-legs", action="store_true",
+                    help="CPU baseline of the headline scene only (no 10k and ensemble legs)")
     ap.add_argument("--walk-shapes", type=str, default=None,
                     help="density,force launch shapes (indices of sphk.walk_shape_names()); "
                          "skips the autotuner")
@@ -73,12 +76,7 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(scene, repeats):
-    """Reference step restated in C (oracle, built with the reference's -O3 -mavx2 -ffast-math
-    -fopenmp), on this host's cores: reset-and-step of step 0, median of `repeats`."""
-    sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import sph_oracle
-
+def _cpu_reset_step(sph_oracle, scene, repeats):
     o = sph_oracle.Oracle(scene.n, scene.oracle_params(sph_oracle), fast=True)
     times = []
     for _ in range(repeats):
@@ -86,10 +84,21 @@ def cpu_baseline(scene, repeats):
         t0 = time.perf_counter()
         o.step(scene.dt)
         times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
     cores = int(o.lib.or_num_threads())
     del o
-    return {
+    return statistics.median(times), cores
+
+
+def cpu_baseline(scene, repeats, sph_scenes, legs=True):
+    """Reference step restated in C (oracle, built with the reference's -O3 -mavx2 -ffast-math
+    -fopenmp), on this host's cores: reset-and-step of step 0, median of `repeats`.  The other
+    legs: the 10k dam break (reset-and-step, and its full 100-step run under the reference
+    formulas) and the 64-scene ensemble stepped scene after scene (step 0 of each)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import sph_oracle
+
+    med, cores = _cpu_reset_step(sph_oracle, scene, repeats)
+    out = {
         "value": scene.n / med,
         "unit": "particle-updates/s",
         "cores": cores,
@@ -98,6 +107,50 @@ def cpu_baseline(scene, repeats):
                   f"{med:.3f} s/step; C restatement of the reference step (hash-map build, "
                   f"125-cell query, per-particle neighbour vectors, OpenMP), reference flags",
     }
+    if not legs:
+        return out
+    c0 = sph_scenes.config0()
+    med0, _ = _cpu_reset_step(sph_oracle, c0, repeats)
+    o = sph_oracle.Oracle(c0.n, c0.oracle_params(sph_oracle), fast=True)
+    o.set_particles(c0.pos, c0.vel)
+    t0 = time.perf_counter()
+    for _ in range(c0.steps):
+        o.step(c0.dt)
+    t_full = time.perf_counter() - t0
+    del o
+    out["dam_break_10k"] = {
+        "reset_and_step": {"value": c0.n / med0, "s_per_step": med0},
+        "full_run": {"steps": c0.steps, "value": c0.n * c0.steps / t_full,
+                     "s_per_step": t_full / c0.steps,
+                     "regime": "reference formulas: the block is thrown into the box corners "
+                               "by step 1 (pile-up, ~N/8 neighbours)"},
+    }
+    ens = sph_scenes.config3(seed=0)
+    tot, t_ens = 0, 0.0
+    for sc in ens:
+        o = sph_oracle.Oracle(sc.n, sc.oracle_params(sph_oracle), fast=True)
+        o.set_particles(sc.pos, sc.vel)
+        t0 = time.perf_counter()
+        o.step(sc.dt)
+        t_ens += time.perf_counter() - t0
+        tot += sc.n
+        del o
+    out["ensemble_64_sequential"] = {"particles": tot, "value": tot / t_ens,
+                                     "s_per_step": t_ens,
+                                     "sample": "step 0 of each of the 64 scenes in turn"}
+    return out
+
+
+def _load_counters(shape_name):
+    """Counter-measured traffic and VALU issue of the force walk, if taken for this shape."""
+    try:
+        with open(COUNTER_PROFILE) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if prof.get("force_shape") != shape_name:
+        return None
+    return prof
 
 
 def time_reset_steps(e, dt, steps, warmup):
@@ -136,6 +189,69 @@ def ensemble_line(sphk, sph_scenes, scale, steps):
            "max_scene": int(max(sc.n for sc in ens)), "ms_per_step": t * 1e3,
            "value": n / t, "unit": "particle-updates/s", "graph": True,
            "walk_shapes": [tune["names"][tune["density"]], tune["names"][tune["force"]]]}
+    del e
+    return out
+
+
+def full_run_line(sphk, scene, steps, dt=1e-4, every=100, damping=None):
+    """`steps` consecutive steps of the 1M dam break under the consistent formulation (the
+    reference formulas pile the block into the box corners within a step), timed in segments
+    of `every` steps; between segments (untimed) the kinetic energy (get_total_energy), the
+    potential energy in the gravity field and the reference's conservation check
+    (compute_conservation_errors: |sum rho h^3 - N m| / N m) are read back."""
+    import numpy as np
+
+    params = scene.params(sphk, formulation=1)
+    if damping is not None:
+        params.damping = damping
+    e = sphk.Engine(scene.n, params)
+    e.upload(scene.pos, scene.vel)
+    p = e.params()
+    m = e.masses().astype(np.float64)
+    g = -float(p.gravity)  # gravity acts on y (compute_external_forces)
+
+    def energies():
+        y = e.positions()[:, 1].astype(np.float64)
+        ke = float(e.total_energy())
+        pe = float(np.sum(m * g * (y - float(p.ymin))))
+        return ke, pe
+
+    traj = []
+    ke, pe = energies()
+    traj.append({"step": 0, "ke": ke, "pe": pe, "e": ke + pe, "mass_error": None})
+    t_steps = 0.0
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        e.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            e.step(dt)
+        e.synchronize()
+        t_steps += time.perf_counter() - t0
+        done += k
+        ke, pe = energies()
+        me, ee = e.conservation_errors()
+        rho = e.densities()[: scene.n].astype(np.float64)
+        traj.append({"step": done, "ke": ke, "pe": pe, "e": ke + pe, "mass_error": float(me),
+                     "energy_error": float(ee), "rho_mean": float(rho.mean()),
+                     "rho_max_dev_frac": float(np.abs(rho - p.rest_density).max() /
+                                               p.rest_density)})
+    e0, e1 = traj[0]["e"], traj[-1]["e"]
+    out = {
+        "workload": f"{scene.name}: {steps} consecutive steps, consistent formulation, dt={dt}, "
+                    f"damping {p.damping:g} per step",
+        "steps": steps, "value": scene.n * steps / t_steps, "unit": "particle-updates/s",
+        "ms_per_step": t_steps / steps * 1e3,
+        "energy_drift_frac": (e1 - e0) / e0,
+        "ke_final": traj[-1]["ke"],
+        "mass_exact": float(m.sum()) == float(np.sum(e.masses().astype(np.float64))),
+        "reference_conservation_check": {"mass_error_final": traj[-1]["mass_error"],
+                                         "energy_error_final": traj[-1]["energy_error"],
+                                         "note": "the reference's check: |sum rho h^3 - N m| "
+                                                 "/ N m, energy_error hard-wired to 0"},
+        "trajectory": traj,
+    }
     del e
     return out
 
@@ -315,6 +431,8 @@ def main():
     ups = n * args.steps / elapsed
     # FP32 pair arithmetic of the two walks (the candidate filter is integer work)
     flops_per_update = FLOP_PER_PAIR * mean_nbrs
+    force_shape = sphk.walk_shape_names()[e.walk_shapes()[1]]
+    counters = _load_counters(force_shape) if args.config == 1 else None
     out = {
         "metric": METRIC,
         "value": ups,
@@ -344,9 +462,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": FORCE_TRAFFIC_BYTES_PER_PARTICLE * n if args.config == 1 else None,
+            "traffic": None if counters is None else counters["hbm_bytes_per_launch"],
+            "traffic_source": None if counters is None else
+            {"profile": os.path.relpath(COUNTER_PROFILE, REPO), "build": counters["build"],
+             "force_shape": counters["force_shape"]},
             "alg_bytes_per_update": ALG_BYTES["force"],
             "kernel_ms": t_force * 1e3,
+            "limiter": "VALU issue and gather latency (HBM traffic is far below its peak)",
+            "valu_busy_frac": None if counters is None else counters["valu_busy_frac"],
         },
         "valu": {
             "fp32_flop_per_update": flops_per_update,
@@ -367,20 +490,10 @@ def main():
     if not args.no_extras:
         out["ensemble"] = ensemble_line(sphk, sph_scenes, args.ensemble_scale, 5)
         out["small_scene_10k"] = graph_line(sphk, sph_scenes, 50)
-    if args.continuous:
-        e2 = sphk.Engine(scene.n, scene.params(sphk, formulation=1))
-        e2.upload(scene.pos, scene.vel)
-        e2.step(1e-4)
-        e2.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.continuous):
-            e2.step(1e-4)
-        e2.synchronize()
-        el = time.perf_counter() - t0
-        out["continuous_consistent"] = {"steps": args.continuous, "dt": 1e-4,
-                                        "value": n * args.continuous / el,
-                                        "ms_per_step": el / args.continuous * 1e3}
-        del e2
+    if args.full_run:
+        out["full_run"] = full_run_line(sphk, scene, args.full_run)
+        undamped = sph_scenes.CONFIGS[args.config]()
+        out["full_run_undamped"] = full_run_line(sphk, undamped, args.full_run, damping=1.0)
     if not args.no_surrogate and args.config == 1:
         out["surrogate"] = surrogate_line(sphk, sph_scenes, args.surrogate_snapshots,
                                           args.surrogate_macro)
@@ -394,7 +507,8 @@ def main():
         torch.cuda.synchronize()
     if not args.no_cpu_baseline:
         del e
-        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_repeats)
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_repeats, sph_scenes,
+                                           legs=not args.no_cpu_legs)
     print(json.dumps(out), flush=True)
 
 

@@ -355,9 +355,9 @@ int or_set_particles(or_state* s, size_t n, const float* pos, const float* vel, 
 
 size_t or_size(const or_state* s) { return s->n; }
 
-/* SPHEngine::update_neighbor_lists (src/sph_engine.cpp:335-353) ->
- * SpatialHash::build (src/spatial_hash.cpp:15-25) + query_squared (:31-57). */
-void or_update_neighbor_lists(or_state* s) {
+/* SpatialHash::build (src/spatial_hash.cpp:15-25): every particle appended, in ascending
+ * index, to the bucket of its cell key. */
+static void build_hash(or_state* s) {
     size_t n = s->n;
     hm_clear(&s->grid);
     hm_init(&s->grid, n / 8 + 16);
@@ -367,8 +367,45 @@ void or_update_neighbor_lists(or_state* s) {
         bucket* b = hm_get_or_insert(&s->grid, or_hash_position(c[0], c[1], c[2]));
         vsz_push(&b->items, i);
     }
+}
+
+/* SpatialHash::query_squared (src/spatial_hash.cpp:31-57) for particle i: the
+ * (2 cr + 1)^3 cells around its cell in (dx, dy, dz) lexicographic order, each bucket in
+ * ascending index.  Appends to `out` when given; returns the count either way. */
+static size_t query_one(const or_state* s, size_t i, vec_sz* out) {
     float r2 = s->p.neighbor_search_radius * s->p.neighbor_search_radius;
     int cr = (int)ceilf(sqrtf(r2) * s->inv_cell) + 1;
+    const float* pi = s->pos + 3 * i;
+    int32_t c[3];
+    or_grid_coords(s->inv_cell, pi, c);
+    size_t cnt = 0;
+    for (int dx = -cr; dx <= cr; ++dx) {
+        for (int dy = -cr; dy <= cr; ++dy) {
+            for (int dz = -cr; dz <= cr; ++dz) {
+                /* ivec3 addition wraps in two's complement */
+                int32_t cx = (int32_t)((uint32_t)c[0] + (uint32_t)dx);
+                int32_t cy = (int32_t)((uint32_t)c[1] + (uint32_t)dy);
+                int32_t cz = (int32_t)((uint32_t)c[2] + (uint32_t)dz);
+                bucket* b = hm_find(&s->grid, or_hash_position(cx, cy, cz));
+                if (!b) continue;
+                for (size_t k = 0; k < b->items.n; ++k) {
+                    size_t j = b->items.v[k];
+                    if (within_r2(pi, s->pos + 3 * j, r2)) {
+                        if (out) vsz_push(out, j);
+                        ++cnt;
+                    }
+                }
+            }
+        }
+    }
+    return cnt;
+}
+
+/* SPHEngine::update_neighbor_lists (src/sph_engine.cpp:335-353) ->
+ * SpatialHash::build (src/spatial_hash.cpp:15-25) + query_squared (:31-57). */
+void or_update_neighbor_lists(or_state* s) {
+    size_t n = s->n;
+    build_hash(s);
     size_t maxn = 0;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) reduction(max : maxn)
@@ -377,28 +414,25 @@ void or_update_neighbor_lists(or_state* s) {
         size_t i = (size_t)ii;
         vec_sz* out = &s->nbr[i];
         out->n = 0;
-        const float* pi = s->pos + 3 * i;
-        int32_t c[3];
-        or_grid_coords(s->inv_cell, pi, c);
-        for (int dx = -cr; dx <= cr; ++dx) {
-            for (int dy = -cr; dy <= cr; ++dy) {
-                for (int dz = -cr; dz <= cr; ++dz) {
-                    /* ivec3 addition wraps in two's complement */
-                    int32_t cx = (int32_t)((uint32_t)c[0] + (uint32_t)dx);
-                    int32_t cy = (int32_t)((uint32_t)c[1] + (uint32_t)dy);
-                    int32_t cz = (int32_t)((uint32_t)c[2] + (uint32_t)dz);
-                    bucket* b = hm_find(&s->grid, or_hash_position(cx, cy, cz));
-                    if (!b) continue;
-                    for (size_t k = 0; k < b->items.n; ++k) {
-                        size_t j = b->items.v[k];
-                        if (within_r2(pi, s->pos + 3 * j, r2)) vsz_push(out, j);
-                    }
-                }
-            }
-        }
+        query_one(s, i, out);
         if (out->n > maxn) maxn = out->n;
     }
     s->max_nbr = maxn;
+}
+
+/* equations::compute_density (src/sph_engine.cpp:370-383) over particle i's list */
+static float density_one(const or_state* s, const kern* k, float w0, size_t i) {
+    const float* pi = s->pos + 3 * i;
+    float density = s->mass[i] * w0;
+    const vec_sz* nb = &s->nbr[i];
+    for (size_t t = 0; t < nb->n; ++t) {
+        size_t j = nb->v[t];
+        if (i == j) continue;
+        const float* pj = s->pos + 3 * j;
+        float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
+        density += s->mass[j] * kW(k, r);
+    }
+    return density;
 }
 
 /* compute_densities + equations::compute_density (src/sph_engine.cpp:203-212,370-383) */
@@ -410,18 +444,7 @@ void or_compute_densities(or_state* s) {
 #pragma omp parallel for schedule(static)
 #endif
     for (long long ii = 0; ii < (long long)s->n; ++ii) {
-        size_t i = (size_t)ii;
-        const float* pi = s->pos + 3 * i;
-        float density = s->mass[i] * w0;
-        const vec_sz* nb = &s->nbr[i];
-        for (size_t t = 0; t < nb->n; ++t) {
-            size_t j = nb->v[t];
-            if (i == j) continue;
-            const float* pj = s->pos + 3 * j;
-            float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-            density += s->mass[j] * kW(&k, r);
-        }
-        s->dens[i] = density;
+        s->dens[ii] = density_one(s, &k, w0, (size_t)ii);
     }
 }
 
@@ -438,55 +461,57 @@ void or_compute_pressures(or_state* s) {
 /* compute_forces (src/sph_engine.cpp:226-244) with compute_pressure_force (:390-412),
  * compute_viscosity_force (:414-432), compute_external_forces (:434-436) and
  * compute_acceleration (:438-443). */
-void or_compute_forces(or_state* s) {
-    kern k = make_kernel(s->p.smoothing_length, s->p.formulation);
+static void force_one(or_state* s, const kern* kp, size_t i) {
+    const kern k = *kp;
     const float visc = s->p.viscosity;
     const float grav = s->p.gravity;
     const int consistent = s->p.formulation == 1;
+    const float* pi = s->pos + 3 * i;
+    const float* vi = s->vel + 3 * i;
+    const vec_sz* nb = &s->nbr[i];
+    float fp[3] = {0.0f, 0.0f, 0.0f};
+    for (size_t t = 0; t < nb->n; ++t) {
+        size_t j = nb->v[t];
+        if (i == j) continue;
+        const float* pj = s->pos + 3 * j;
+        float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
+        float r_len = vlen(r);
+        if (r_len < 1e-6f) continue;
+        float pressure_term = (s->pres[i] + s->pres[j]) / (2.0f * s->dens[j]);
+        float g[3];
+        kgradW(&k, r, g);
+        float c = s->mass[j] * pressure_term;
+        fp[0] = fp[0] - c * g[0];
+        fp[1] = fp[1] - c * g[1];
+        fp[2] = fp[2] - c * g[2];
+    }
+    float fv[3] = {0.0f, 0.0f, 0.0f};
+    for (size_t t = 0; t < nb->n; ++t) {
+        size_t j = nb->v[t];
+        if (i == j) continue;
+        const float* pj = s->pos + 3 * j;
+        const float* vj = s->vel + 3 * j;
+        float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
+        float vij[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
+        float lap = klapW(&k, r);
+        float c = (s->mass[j] / s->dens[j]) * visc;
+        fv[0] = fv[0] + (c * vij[0]) * lap;
+        fv[1] = fv[1] + (c * vij[1]) * lap;
+        fv[2] = fv[2] + (c * vij[2]) * lap;
+    }
+    float ext[3] = {0.0f, grav, 0.0f};
+    float m = consistent ? s->dens[i] : s->mass[i];
+    for (int d = 0; d < 3; ++d) {
+        s->acc[3 * i + d] = ((fp[d] + fv[d]) + ext[d] * m) / m;
+    }
+}
+
+void or_compute_forces(or_state* s) {
+    kern k = make_kernel(s->p.smoothing_length, s->p.formulation);
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static)
 #endif
-    for (long long ii = 0; ii < (long long)s->n; ++ii) {
-        size_t i = (size_t)ii;
-        const float* pi = s->pos + 3 * i;
-        const float* vi = s->vel + 3 * i;
-        const vec_sz* nb = &s->nbr[i];
-        float fp[3] = {0.0f, 0.0f, 0.0f};
-        for (size_t t = 0; t < nb->n; ++t) {
-            size_t j = nb->v[t];
-            if (i == j) continue;
-            const float* pj = s->pos + 3 * j;
-            float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-            float r_len = vlen(r);
-            if (r_len < 1e-6f) continue;
-            float pressure_term = (s->pres[i] + s->pres[j]) / (2.0f * s->dens[j]);
-            float g[3];
-            kgradW(&k, r, g);
-            float c = s->mass[j] * pressure_term;
-            fp[0] = fp[0] - c * g[0];
-            fp[1] = fp[1] - c * g[1];
-            fp[2] = fp[2] - c * g[2];
-        }
-        float fv[3] = {0.0f, 0.0f, 0.0f};
-        for (size_t t = 0; t < nb->n; ++t) {
-            size_t j = nb->v[t];
-            if (i == j) continue;
-            const float* pj = s->pos + 3 * j;
-            const float* vj = s->vel + 3 * j;
-            float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-            float vij[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
-            float lap = klapW(&k, r);
-            float c = (s->mass[j] / s->dens[j]) * visc;
-            fv[0] = fv[0] + (c * vij[0]) * lap;
-            fv[1] = fv[1] + (c * vij[1]) * lap;
-            fv[2] = fv[2] + (c * vij[2]) * lap;
-        }
-        float ext[3] = {0.0f, grav, 0.0f};
-        float m = consistent ? s->dens[i] : s->mass[i];
-        for (int d = 0; d < 3; ++d) {
-            s->acc[3 * i + d] = ((fp[d] + fv[d]) + ext[d] * m) / m;
-        }
-    }
+    for (long long ii = 0; ii < (long long)s->n; ++ii) force_one(s, &k, (size_t)ii);
 }
 
 static inline void integrate_one(float* x, float* v, const float* a, float dt, float damping) {
@@ -657,6 +682,79 @@ static double L64(const kern64* k, double q) {
     return 0.0;
 }
 
+/* FP64 density of particle i over its list (self skipped, added as m_i W(0)). */
+static double dens64_one(const kern64* k, size_t i, const float* pos, const float* mass,
+                         const int64_t* list, int64_t len) {
+    double d = (double)mass[i] * W64(k, 0.0);
+    for (int64_t t = 0; t < len; ++t) {
+        size_t j = (size_t)list[t];
+        if (j == i) continue;
+        double r[3] = {(double)pos[3 * i] - pos[3 * j], (double)pos[3 * i + 1] - pos[3 * j + 1],
+                       (double)pos[3 * i + 2] - pos[3 * j + 2]};
+        double rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        d += (double)mass[j] * W64(k, rl / k->h);
+    }
+    return d;
+}
+
+/* FP64 acceleration of particle i over its list, densities rho (indexed like pos). */
+static void acc64_one(const kern64* k, size_t i, const float* pos, const float* vel,
+                      const float* mass, const int64_t* list, int64_t len, const double* rho,
+                      const or_params* p, double* acc) {
+    double Pi = (double)p->gas_constant * (rho[i] - (double)p->rest_density);
+    double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0};
+    for (int64_t t = 0; t < len; ++t) {
+        size_t j = (size_t)list[t];
+        if (j == i) continue;
+        double r[3] = {(double)pos[3 * i] - pos[3 * j], (double)pos[3 * i + 1] - pos[3 * j + 1],
+                       (double)pos[3 * i + 2] - pos[3 * j + 2]};
+        double rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        double q = rl / k->h;
+        double Pj = (double)p->gas_constant * (rho[j] - (double)p->rest_density);
+        if (rl >= 1e-6) {
+            double pt = (Pi + Pj) / (2.0 * rho[j]);
+            double gs = dW64(k, q) / (rl * k->h);
+            for (int c = 0; c < 3; ++c) fp[c] -= (double)mass[j] * pt * gs * r[c];
+        }
+        double lap = L64(k, q);
+        double cc = (double)mass[j] / rho[j] * (double)p->viscosity * lap;
+        for (int c = 0; c < 3; ++c)
+            fv[c] += cc * ((double)vel[3 * j + c] - (double)vel[3 * i + c]);
+    }
+    double m = p->formulation == 1 ? rho[i] : (double)mass[i];
+    double ext[3] = {0.0, (double)p->gravity, 0.0};
+    for (int c = 0; c < 3; ++c) acc[c] = (fp[c] + fv[c] + ext[c] * m) / m;
+}
+
+/* Sum of the magnitudes of particle i's pressure and viscosity terms over its list, divided
+ * by m_i (rho_i in the consistent formulation), plus |g|. */
+static double scale_one(const kern64* k, size_t i, const float* pos, const float* vel,
+                        const float* mass, const float* rho, const int64_t* list, int64_t len,
+                        const or_params* p) {
+    double Pi = (double)p->gas_constant * ((double)rho[i] - (double)p->rest_density);
+    double sum = 0.0;
+    for (int64_t t = 0; t < len; ++t) {
+        size_t j = (size_t)list[t];
+        if (j == i) continue;
+        double r[3] = {(double)pos[3 * i] - pos[3 * j], (double)pos[3 * i + 1] - pos[3 * j + 1],
+                       (double)pos[3 * i + 2] - pos[3 * j + 2]};
+        double rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        double q = rl / k->h;
+        double Pj = (double)p->gas_constant * ((double)rho[j] - (double)p->rest_density);
+        if (rl >= 1e-6) {
+            double pt = (Pi + Pj) / (2.0 * (double)rho[j]);
+            sum += fabs((double)mass[j] * pt * dW64(k, q) / k->h);
+        }
+        double lap = L64(k, q);
+        double dv[3] = {(double)vel[3 * j] - vel[3 * i], (double)vel[3 * j + 1] - vel[3 * i + 1],
+                        (double)vel[3 * j + 2] - vel[3 * i + 2]};
+        double dvl = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+        sum += fabs((double)mass[j] / (double)rho[j] * (double)p->viscosity * lap) * dvl;
+    }
+    double m = p->formulation == 1 ? (double)rho[i] : (double)mass[i];
+    return sum / fabs(m) + fabs((double)p->gravity);
+}
+
 void or64_density_accel(size_t n, const float* pos, const float* vel, const float* mass,
                         const int64_t* offsets, const int64_t* idx, const or_params* p,
                         double* rho, double* acc) {
@@ -666,16 +764,7 @@ void or64_density_accel(size_t n, const float* pos, const float* vel, const floa
 #endif
     for (long long ii = 0; ii < (long long)n; ++ii) {
         size_t i = (size_t)ii;
-        double d = (double)mass[i] * W64(&k, 0.0);
-        for (int64_t t = offsets[i]; t < offsets[i + 1]; ++t) {
-            size_t j = (size_t)idx[t];
-            if (j == i) continue;
-            double r[3] = {(double)pos[3 * i] - pos[3 * j], (double)pos[3 * i + 1] - pos[3 * j + 1],
-                           (double)pos[3 * i + 2] - pos[3 * j + 2]};
-            double rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-            d += (double)mass[j] * W64(&k, rl / k.h);
-        }
-        rho[i] = d;
+        rho[i] = dens64_one(&k, i, pos, mass, idx + offsets[i], offsets[i + 1] - offsets[i]);
     }
     if (!acc) return;
 #ifdef _OPENMP
@@ -683,29 +772,8 @@ void or64_density_accel(size_t n, const float* pos, const float* vel, const floa
 #endif
     for (long long ii = 0; ii < (long long)n; ++ii) {
         size_t i = (size_t)ii;
-        double Pi = (double)p->gas_constant * (rho[i] - (double)p->rest_density);
-        double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0};
-        for (int64_t t = offsets[i]; t < offsets[i + 1]; ++t) {
-            size_t j = (size_t)idx[t];
-            if (j == i) continue;
-            double r[3] = {(double)pos[3 * i] - pos[3 * j], (double)pos[3 * i + 1] - pos[3 * j + 1],
-                           (double)pos[3 * i + 2] - pos[3 * j + 2]};
-            double rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-            double q = rl / k.h;
-            double Pj = (double)p->gas_constant * (rho[j] - (double)p->rest_density);
-            if (rl >= 1e-6) {
-                double pt = (Pi + Pj) / (2.0 * rho[j]);
-                double gs = dW64(&k, q) / (rl * k.h);
-                for (int c = 0; c < 3; ++c) fp[c] -= (double)mass[j] * pt * gs * r[c];
-            }
-            double lap = L64(&k, q);
-            double cc = (double)mass[j] / rho[j] * (double)p->viscosity * lap;
-            for (int c = 0; c < 3; ++c)
-                fv[c] += cc * ((double)vel[3 * j + c] - (double)vel[3 * i + c]);
-        }
-        double m = p->formulation == 1 ? rho[i] : (double)mass[i];
-        double ext[3] = {0.0, (double)p->gravity, 0.0};
-        for (int c = 0; c < 3; ++c) acc[3 * i + c] = (fp[c] + fv[c] + ext[c] * m) / m;
+        acc64_one(&k, i, pos, vel, mass, idx + offsets[i], offsets[i + 1] - offsets[i], rho, p,
+                  acc + 3 * i);
     }
 }
 
@@ -718,29 +786,152 @@ void or64_accel_term_scale(size_t n, const float* pos, const float* vel, const f
 #endif
     for (long long ii = 0; ii < (long long)n; ++ii) {
         size_t i = (size_t)ii;
-        double Pi = (double)p->gas_constant * ((double)rho[i] - (double)p->rest_density);
-        double sum = 0.0;
-        for (int64_t t = offsets[i]; t < offsets[i + 1]; ++t) {
-            size_t j = (size_t)idx[t];
-            if (j == i) continue;
-            double r[3] = {(double)pos[3 * i] - pos[3 * j], (double)pos[3 * i + 1] - pos[3 * j + 1],
-                           (double)pos[3 * i + 2] - pos[3 * j + 2]};
-            double rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-            double q = rl / k.h;
-            double Pj = (double)p->gas_constant * ((double)rho[j] - (double)p->rest_density);
-            if (rl >= 1e-6) {
-                double pt = (Pi + Pj) / (2.0 * (double)rho[j]);
-                sum += fabs((double)mass[j] * pt * dW64(&k, q) / k.h);
-            }
-            double lap = L64(&k, q);
-            double dv[3] = {(double)vel[3 * j] - vel[3 * i], (double)vel[3 * j + 1] - vel[3 * i + 1],
-                            (double)vel[3 * j + 2] - vel[3 * i + 2]};
-            double dvl = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
-            sum += fabs((double)mass[j] / (double)rho[j] * (double)p->viscosity * lap) * dvl;
-        }
-        double m = p->formulation == 1 ? (double)rho[i] : (double)mass[i];
-        scale[i] = sum / fabs(m) + fabs((double)p->gravity);
+        scale[i] = scale_one(&k, i, pos, vel, mass, rho, idx + offsets[i],
+                             offsets[i + 1] - offsets[i], p);
     }
+}
+
+/* ---------------------------------------------------------------------------
+ * Sampled evaluation for scenes too large for a neighbour vector per particle.
+ * ------------------------------------------------------------------------- */
+uint64_t or_count_neighbors(or_state* s, uint64_t* max_out) {
+    build_hash(s);
+    uint64_t total = 0, mx = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : total) reduction(max : mx)
+#endif
+    for (long long ii = 0; ii < (long long)s->n; ++ii) {
+        uint64_t c = (uint64_t)query_one(s, (size_t)ii, NULL);
+        total += c;
+        if (c > mx) mx = c;
+    }
+    if (max_out) *max_out = mx;
+    return total;
+}
+
+int or_sample_step0(or_state* s, size_t ns, const int64_t* sample, int64_t* counts, float* rho,
+                    float* acc, double* rho64, double* acc64, double* scale) {
+    const size_t n = s->n;
+    for (size_t k = 0; k < ns; ++k)
+        if (sample[k] < 0 || (size_t)sample[k] >= n) return -1;
+    build_hash(s);
+    /* need: bit 0 density (the sample and its neighbours), bit 1 sample, bit 2 queried */
+    uint8_t* need = (uint8_t*)calloc(n ? n : 1, 1);
+    size_t* uniq = (size_t*)malloc((ns ? ns : 1) * sizeof(size_t));
+    size_t nu = 0;
+    for (size_t k = 0; k < ns; ++k) {
+        size_t i = (size_t)sample[k];
+        if (!(need[i] & 2)) uniq[nu++] = i;
+        need[i] |= 3;
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (long long u = 0; u < (long long)nu; ++u) {
+        size_t i = uniq[u];
+        s->nbr[i].n = 0;
+        query_one(s, i, &s->nbr[i]);
+    }
+    for (size_t u = 0; u < nu; ++u) {
+        const vec_sz* nb = &s->nbr[uniq[u]];
+        need[uniq[u]] |= 4;
+        for (size_t t = 0; t < nb->n; ++t) need[nb->v[t]] |= 1;
+    }
+    size_t na = 0;
+    for (size_t i = 0; i < n; ++i) na += need[i] & 1;
+    size_t* A = (size_t*)malloc((na ? na : 1) * sizeof(size_t));
+    na = 0;
+    for (size_t i = 0; i < n; ++i)
+        if (need[i] & 1) A[na++] = i;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+    for (long long a = 0; a < (long long)na; ++a) {
+        size_t i = A[a];
+        if (need[i] & 4) continue;
+        s->nbr[i].n = 0;
+        query_one(s, i, &s->nbr[i]);
+    }
+    kern k = make_kernel(s->p.smoothing_length, s->p.formulation);
+    float zero[3] = {0.0f, 0.0f, 0.0f};
+    float w0 = kW(&k, zero);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+    for (long long a = 0; a < (long long)na; ++a) {
+        size_t i = A[a];
+        s->dens[i] = density_one(s, &k, w0, i);
+        s->pres[i] = s->p.gas_constant * (s->dens[i] - s->p.rest_density);
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (long long u = 0; u < (long long)nu; ++u) force_one(s, &k, uniq[u]);
+    /* FP64 over the same lists: densities of A, then the samples' accelerations */
+    kern64 k64 = make_kernel64((double)s->p.smoothing_length, s->p.formulation);
+    double* r64 = NULL;
+    if (rho64 || acc64) {
+        r64 = (double*)calloc(n ? n : 1, sizeof(double));
+        int64_t** lists = (int64_t**)calloc(na ? na : 1, sizeof(int64_t*));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+        for (long long a = 0; a < (long long)na; ++a) {
+            size_t i = A[a];
+            const vec_sz* nb = &s->nbr[i];
+            int64_t* l = (int64_t*)malloc((nb->n ? nb->n : 1) * sizeof(int64_t));
+            for (size_t t = 0; t < nb->n; ++t) l[t] = (int64_t)nb->v[t];
+            r64[i] = dens64_one(&k64, i, s->pos, s->mass, l, (int64_t)nb->n);
+            lists[a] = l;
+        }
+        for (size_t a = 0; a < na; ++a) free(lists[a]);
+        free(lists);
+    }
+    for (size_t kk = 0; kk < ns; ++kk) {
+        size_t i = (size_t)sample[kk];
+        const vec_sz* nb = &s->nbr[i];
+        if (counts) counts[kk] = (int64_t)nb->n;
+        if (rho) rho[kk] = s->dens[i];
+        if (acc)
+            for (int d = 0; d < 3; ++d) acc[3 * kk + d] = s->acc[3 * i + d];
+        if (!rho64 && !acc64 && !scale) continue;
+        int64_t* l = (int64_t*)malloc((nb->n ? nb->n : 1) * sizeof(int64_t));
+        for (size_t t = 0; t < nb->n; ++t) l[t] = (int64_t)nb->v[t];
+        if (rho64) rho64[kk] = r64[i];
+        if (acc64) acc64_one(&k64, i, s->pos, s->vel, s->mass, l, (int64_t)nb->n, r64, &s->p,
+                             acc64 + 3 * kk);
+        if (scale)
+            scale[kk] = scale_one(&k64, i, s->pos, s->vel, s->mass, s->dens, l, (int64_t)nb->n,
+                                  &s->p);
+        free(l);
+    }
+    /* the lists of the non-sample particles are dropped; the samples' stay readable through
+     * or_sample_neighbors */
+    for (size_t a = 0; a < na; ++a) {
+        size_t i = A[a];
+        if (need[i] & 2) continue;
+        free(s->nbr[i].v);
+        s->nbr[i].v = NULL;
+        s->nbr[i].n = s->nbr[i].cap = 0;
+    }
+    free(r64);
+    free(A);
+    free(uniq);
+    free(need);
+    return 0;
+}
+
+void or_sample_neighbors(const or_state* s, size_t ns, const int64_t* sample, int64_t* offsets,
+                         int64_t* idx) {
+    int64_t o = 0;
+    for (size_t k = 0; k < ns; ++k) {
+        const vec_sz* nb = &s->nbr[(size_t)sample[k]];
+        offsets[k] = o;
+        if (idx)
+            for (size_t t = 0; t < nb->n; ++t) idx[o + (int64_t)t] = (int64_t)nb->v[t];
+        o += (int64_t)nb->n;
+    }
+    offsets[ns] = o;
 }
 
 size_t or_brute_neighbors(size_t n, const float* pos, float r2, int64_t* offsets, int64_t* idx,

@@ -111,6 +111,22 @@ void      or64_density_accel(size_t n, const float* pos, const float* vel, const
 void      or64_accel_term_scale(size_t n, const float* pos, const float* vel, const float* mass,
                                 const float* rho, const int64_t* offsets, const int64_t* idx,
                                 const or_params* p, double* scale);
+/* Sampled evaluation for scenes too large for a neighbour vector per particle.
+ * or_count_neighbors: builds the hash over all particles and runs the query of every
+ *   particle counting only; returns the sum of the list lengths, the longest in *max_out.
+ * or_sample_step0: builds the hash over all particles, queries the sampled particles and
+ *   their neighbours, and evaluates densities (FP32, for the sample and its neighbours) and
+ *   accelerations (FP32, for the sample) exactly as the full phases would; per sample k:
+ *   list length, density, acceleration (3), and over the same lists the FP64 density,
+ *   FP64 acceleration (3) and the conditioning scale of or64_accel_term_scale.  Any output
+ *   may be NULL.  Returns -1 for an index outside [0, n).
+ * or_sample_neighbors: the lists of the sample after or_sample_step0 (CSR, ns + 1 offsets;
+ *   idx NULL for the offsets only). */
+uint64_t  or_count_neighbors(or_state* s, uint64_t* max_out);
+int       or_sample_step0(or_state* s, size_t ns, const int64_t* sample, int64_t* counts,
+                          float* rho, float* acc, double* rho64, double* acc64, double* scale);
+void      or_sample_neighbors(const or_state* s, size_t ns, const int64_t* sample,
+                              int64_t* offsets, int64_t* idx);
 /* Brute-force O(n^2) neighbour sets with the same d^2 operation order (ascending j). */
 size_t    or_brute_neighbors(size_t n, const float* pos, float r2, int64_t* offsets, int64_t* idx,
                              size_t cap);

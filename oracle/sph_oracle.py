@@ -80,6 +80,9 @@ _SIG = {
     "or64_density_accel": (None, [_SZ, _F, _F, _F, _I64, _I64, C.POINTER(Params), _D, _D]),
     "or64_accel_term_scale": (None, [_SZ, _F, _F, _F, _F, _I64, _I64, C.POINTER(Params), _D]),
     "or_brute_neighbors": (_SZ, [_SZ, _F, C.c_float, _I64, _I64, _SZ]),
+    "or_count_neighbors": (C.c_uint64, [_P, _U64]),
+    "or_sample_step0": (C.c_int, [_P, _SZ, _I64, _I64, _F, _F, _D, _D, _D]),
+    "or_sample_neighbors": (None, [_P, _SZ, _I64, _I64, _I64]),
     "or_num_threads": (C.c_int, []),
     "or_bf16_round": (C.c_float, [C.c_float]),
     "or_sur_nparams": (_SZ, [C.POINTER(SurCfg)]),
@@ -174,6 +177,41 @@ class Oracle:
         idx = np.empty(max(tot, 1), dtype=np.int64)
         self.lib.or_get_neighbors(self.h, offs.ctypes.data_as(_I64), idx.ctypes.data_as(_I64))
         return offs, idx[:tot]
+
+    def count_neighbors(self):
+        """(sum of list lengths, longest list) of the current state, lists not stored."""
+        mx = C.c_uint64(0)
+        tot = self.lib.or_count_neighbors(self.h, C.byref(mx))
+        return int(tot), int(mx.value)
+
+    def sample_step0(self, sample):
+        """Step-0 phases evaluated for the sampled particles only (and the densities of their
+        neighbours, which their forces read): per sample the list length, FP32 density and
+        acceleration, FP64 density and acceleration over the same lists, and the scale S_i."""
+        sample = np.ascontiguousarray(sample, dtype=np.int64)
+        k = sample.shape[0]
+        out = dict(counts=np.empty(k, np.int64), rho=np.empty(k, np.float32),
+                   acc=np.empty((k, 3), np.float32), rho64=np.empty(k, np.float64),
+                   acc64=np.empty((k, 3), np.float64), scale=np.empty(k, np.float64))
+        rc = self.lib.or_sample_step0(
+            self.h, k, sample.ctypes.data_as(_I64), out["counts"].ctypes.data_as(_I64),
+            _f(out["rho"]), _f(out["acc"]), out["rho64"].ctypes.data_as(_D),
+            out["acc64"].ctypes.data_as(_D), out["scale"].ctypes.data_as(_D))
+        if rc != 0:
+            raise ValueError("sample index out of range")
+        return out
+
+    def sample_neighbors(self, sample):
+        """Neighbour lists (CSR) of the sample after sample_step0."""
+        sample = np.ascontiguousarray(sample, dtype=np.int64)
+        k = sample.shape[0]
+        offs = np.empty(k + 1, np.int64)
+        self.lib.or_sample_neighbors(self.h, k, sample.ctypes.data_as(_I64),
+                                     offs.ctypes.data_as(_I64), None)
+        idx = np.empty(max(int(offs[-1]), 1), np.int64)
+        self.lib.or_sample_neighbors(self.h, k, sample.ctypes.data_as(_I64),
+                                     offs.ctypes.data_as(_I64), idx.ctypes.data_as(_I64))
+        return offs, idx[: int(offs[-1])]
 
 
 def cell_keys(pos: np.ndarray, cell: float):

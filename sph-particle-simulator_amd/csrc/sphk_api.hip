@@ -93,6 +93,10 @@ StepConsts make_consts(const sphk_params& p) {
     c.grav = p.gravity;
     c.damping = p.damping;
     c.formulation = p.formulation;
+    // the reference counts a pair only when d^2 <= r^2 (src/sph_engine.cpp:347,
+    // src/spatial_hash.h:70-73); with a radius below the support 2h a pair beyond the radius
+    // would still carry kernel weight, so the walks then test it
+    c.cut = p.neighbor_search_radius < 2.0f * h ? 1 : 0;
     // Quantized candidate filter.  X = floor(x * qscale) per axis (FP64) differs from
     // (x_a - x_b) qscale by under 1 + 2^-12, so a pair that passes the FP32 test (true
     // distance below sqrt(r2) (1 + 3 * 2^-24)) has |dX| < bound and dX^2 + dY^2 + dZ^2 <= T.
@@ -361,9 +365,14 @@ int ensure_grid(sphk_engine* e) {
     const uint64_t chunks = (e->cap + 63) / 64 + ncols;
     WalkBuffers& wb = e->walk;
     if (chunks > wb.chunk_cap) {
-        for (uint32_t* p : {wb.items, wb.mask, wb.wj0, wb.nwords, wb.pstarts})
-            if (p) HIP_TRY(hipFree(p));
-        size_t want = std::max<size_t>(chunks, wb.chunk_cap * 5 / 4);
+        // every member is cleared as it is freed, so a failed allocation below leaves no
+        // freed pointer behind (the next call or sphk_destroy frees only live ones)
+        for (uint32_t** p : {&wb.items, &wb.mask, &wb.wj0, &wb.nwords, &wb.pstarts}) {
+            if (*p) HIP_TRY(hipFree(*p));
+            *p = nullptr;
+        }
+        const size_t want = std::max<size_t>(chunks, wb.chunk_cap * 5 / 4);
+        wb.chunk_cap = 0;
         HIP_TRY(dalloc(&wb.items, want));
         HIP_TRY(dalloc(&wb.mask, want * kMaxWords * 64));
         HIP_TRY(dalloc(&wb.wj0, want * kMaxWords));
@@ -377,8 +386,13 @@ int ensure_grid(sphk_engine* e) {
     // per-part sums of the split walk shapes: three sums per slot, only while selected
     if ((walk_shape_split(e->shape_density) || walk_shape_split(e->shape_force)) &&
         wb.part_cap < e->cap) {
-        for (void* p : {(void*)wb.part_acc, (void*)wb.part_cnt, (void*)wb.part_force})
-            if (p) HIP_TRY(hipFree(p));
+        if (wb.part_acc) HIP_TRY(hipFree(wb.part_acc));
+        wb.part_acc = nullptr;
+        if (wb.part_cnt) HIP_TRY(hipFree(wb.part_cnt));
+        wb.part_cnt = nullptr;
+        if (wb.part_force) HIP_TRY(hipFree(wb.part_force));
+        wb.part_force = nullptr;
+        wb.part_cap = 0;
         HIP_TRY(dalloc(&wb.part_acc, kParts * e->cap));
         HIP_TRY(dalloc(&wb.part_cnt, kParts * e->cap));
         HIP_TRY(dalloc(&wb.part_force, 2 * kParts * e->cap));

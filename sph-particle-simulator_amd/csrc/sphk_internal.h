@@ -32,6 +32,9 @@ struct StepConsts {
     float lap_scale;   // 2 sigma * mu / h^2   (applied once per particle)
     float k_gas, rho0, grav, damping;
     int32_t formulation;
+    int32_t cut;      // 1 when the search radius is below the kernel support 2h: the walks
+                      // then drop pairs beyond the radius explicitly (else their weight is
+                      // zero or below 2^-75 of a neighbour's)
     uint32_t G;       // cells of all scene grids; particles with dense index G have a
                       // non-finite coordinate
     uint32_t ncols;   // (x, y) columns of all scene grids

@@ -102,19 +102,24 @@ __device__ __forceinline__ float w6_shape8(float r, float half_inv_h, float inv_
 }
 
 // One neighbour's m_j (6 W / 8 sigma).  |r| comes from the reference's d^2, which the
-// neighbour count (on every step of the engine) needs anyway.
-template <bool kCount>
+// neighbour count (on every step of the engine) needs anyway.  Pairs the filter let through
+// beyond the radius: with a radius of at least 2h they have q >= 2 (up to the rounding of
+// sqrt, which leaves a shape below 2^-75) and weigh nothing; with a smaller radius (kCut) the
+// reference's test d^2 <= r^2 drops them explicitly.
+template <bool kCount, bool kCut>
 __device__ __forceinline__ float density_pair(const float4& pi, const float4& q,
                                               const StepConsts& c, float half_inv_h, float acc,
                                               uint32_t& cnt) {
     const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
-    if (kCount) cnt += d2 <= c.r2 ? 1u : 0u;  // the reference's test
-    // pairs the filter let through beyond the radius have q >= 2 and weigh zero
-    return __builtin_fmaf(q.w, w6_shape8(__builtin_amdgcn_sqrtf(d2), half_inv_h, c.inv_h), acc);
+    const bool in = d2 <= c.r2;  // the reference's test
+    if (kCount) cnt += in ? 1u : 0u;
+    float w = w6_shape8(__builtin_amdgcn_sqrtf(d2), half_inv_h, c.inv_h);
+    if (kCut) w = in ? w : 0.0f;
+    return __builtin_fmaf(q.w, w, acc);
 }
 
 // Neighbour positions are requested two steps ahead, in three rotating register sets.
-template <bool kCount, int kSeg>
+template <bool kCount, bool kCut, int kSeg>
 __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                               const uint32_t* __restrict__ lmask,
                                               const uint32_t* __restrict__ lwj0, uint32_t lane,
@@ -127,14 +132,14 @@ __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
     uint32_t t = 0;
     for (; t + 3 <= steps; t += 3) {
         const float4 q2 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
-        acc = density_pair<kCount>(pi, q0, c, hh, acc, cnt);
+        acc = density_pair<kCount, kCut>(pi, q0, c, hh, acc, cnt);
         q0 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
-        acc = density_pair<kCount>(pi, q1, c, hh, acc, cnt);
+        acc = density_pair<kCount, kCut>(pi, q1, c, hh, acc, cnt);
         q1 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
-        acc = density_pair<kCount>(pi, q2, c, hh, acc, cnt);
+        acc = density_pair<kCount, kCut>(pi, q2, c, hh, acc, cnt);
     }
-    if (t < steps) acc = density_pair<kCount>(pi, q0, c, hh, acc, cnt);
-    if (t + 1 < steps) acc = density_pair<kCount>(pi, q1, c, hh, acc, cnt);
+    if (t < steps) acc = density_pair<kCount, kCut>(pi, q0, c, hh, acc, cnt);
+    if (t + 1 < steps) acc = density_pair<kCount, kCut>(pi, q1, c, hh, acc, cnt);
     return acc;
 }
 
@@ -158,7 +163,7 @@ __device__ __forceinline__ void density_epilogue(float sum, const float4& pi, ui
 // three wavefronts per chunk, the sums finished by k_density_finish).  Seven wavefronts per
 // SIMD: the 15-word shapes' LDS allows it, and the walk is VALU-bound (a few spills outside
 // the walk loops; measured faster at 1M than the 6 the default register allocation gives).
-template <class Shape, bool kSplit>
+template <class Shape, bool kSplit, bool kCut>
 __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_density_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const uint2* __restrict__ sq, float4* __restrict__ svr,
@@ -220,9 +225,9 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
         bw.begin(sl.nonempty, lmask, lwj0, lane);
         const uint32_t steps = wave_max(sl.count);
         if (stats || kSplit)
-            cur = density_bits<true>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur, cnt);
+            cur = density_bits<true, kCut>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur, cnt);
         else
-            cur = density_bits<false>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur, cnt);
+            cur = density_bits<false, kCut>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur, cnt);
         seg_base += nw;
         wave_sync();
         if (tab ? ti >= wt.n : cu.done) break;
@@ -323,15 +328,25 @@ __device__ __forceinline__ float3 final_accel(float3 fp, float3 fv, float m_i, f
 }
 
 // Pressure and viscosity sums of one target over neighbour rows (ForceRow a, b).
-// Pairs the filter let through beyond the radius (q >= 2), the self pair (r = 0, v_j = v_i)
-// and the padding slot (B = C = 0) all add exactly zero.
+// The self pair (r = 0, v_j = v_i) and the padding slot (B = C = 0) add exactly zero.  Pairs
+// the filter let through beyond the radius: with a radius of at least 2h they have q >= 2 (up
+// to rounding, which leaves a Laplacian factor below 3e-8 of a neighbour's) and add nothing;
+// with a smaller radius (kCut) the reference's test on its own d^2 drops them: d^2 is pushed
+// to 1e30, where both kernel factors clamp to zero.
 struct ForceSums {
     float3 fp = make_float3(0.f, 0.f, 0.f);
     float3 fv = make_float3(0.f, 0.f, 0.f);
+    template <bool kCut>
     __device__ __forceinline__ void add(const float4& pi, const float3& vi, float p_quarter,
                                         const float4& a, const float4& b, const StepConsts& c) {
         const float rx = pi.x - a.x, ry = pi.y - a.y, rz = pi.z - a.z;
-        const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+        float d2;
+        if (kCut) {
+            d2 = dist2_exact(pi.x, pi.y, pi.z, a.x, a.y, a.z);
+            d2 = d2 <= c.r2 ? d2 : 1e30f;
+        } else {
+            d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+        }
         const float rinv = __builtin_amdgcn_rsqf(fmaxf(d2, 1e-30f));
         const float rl = d2 * rinv;
         // u = (2 - q) / 2 and s = 1 - q, clamped to [0, 1] by the instructions' clamp bit
@@ -359,7 +374,7 @@ struct ForceSums {
 // The wavefront runs its largest lane count; rows are requested two neighbours ahead, in
 // three rotating register sets (a gather from L2 takes several hundred cycles, one step of
 // pair arithmetic about a hundred).
-template <int kSeg>
+template <int kSeg, bool kCut>
 __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                            const uint32_t* __restrict__ lmask,
                                            const uint32_t* __restrict__ lwj0, uint32_t lane,
@@ -376,18 +391,18 @@ __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
     for (; t + 3 <= steps; t += 3) {
         o = bw.template next<5>(lmask, lwj0, lane);
         const float4 a2 = row_at(rows, o).a, b2 = row_at(rows, o).b;
-        f.add(pi, vi, p_quarter, a0, b0, c);
+        f.template add<kCut>(pi, vi, p_quarter, a0, b0, c);
         o = bw.template next<5>(lmask, lwj0, lane);
         a0 = row_at(rows, o).a;
         b0 = row_at(rows, o).b;
-        f.add(pi, vi, p_quarter, a1, b1, c);
+        f.template add<kCut>(pi, vi, p_quarter, a1, b1, c);
         o = bw.template next<5>(lmask, lwj0, lane);
         a1 = row_at(rows, o).a;
         b1 = row_at(rows, o).b;
-        f.add(pi, vi, p_quarter, a2, b2, c);
+        f.template add<kCut>(pi, vi, p_quarter, a2, b2, c);
     }
-    if (t < steps) f.add(pi, vi, p_quarter, a0, b0, c);
-    if (t + 1 < steps) f.add(pi, vi, p_quarter, a1, b1, c);
+    if (t < steps) f.template add<kCut>(pi, vi, p_quarter, a0, b0, c);
+    if (t + 1 < steps) f.template add<kCut>(pi, vi, p_quarter, a1, b1, c);
 }
 
 // Folds the sums of finished parts into the running total, which starts at +0:
@@ -448,7 +463,7 @@ __device__ __forceinline__ void force_epilogue(const ForceSums& f, float4 pi, fl
 // three wavefronts per chunk, finished by k_force_finish).  Shapes of 24 words and more keep
 // the parts' running total in registers: their LDS, not their registers, limits occupancy,
 // and the 1.5 KB per wavefront it frees lets the 28-word shape run 5 wavefronts per SIMD.
-template <class Shape, bool kSplit>
+template <class Shape, bool kSplit, bool kCut>
 __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(Shape::force_waves)))
 void k_force_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
@@ -550,7 +565,7 @@ void k_force_walk(
         wave_sync();
         BitWalk<kSeg> bw;
         bw.begin(sl.nonempty, lmask, lwj0, lane);
-        force_bits<kSeg>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi, p_quarter, c,
+        force_bits<kSeg, kCut>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi, p_quarter, c,
                          f);
         seg_base += nw;
         wave_sync();
@@ -766,12 +781,19 @@ hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const u
     PartSums ps;
     if (hipError_t err = part_sums(wb, n, shape, &ps)) return err;
     const uint32_t chunks = max_chunks(n, c);
-    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL((k_density_walk<S, kS>),
-                                                  dim3(grid_for<S>(chunks, kS)),
-                                                  dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, sq, svr, rows, skey, cell_start, c,
-                                                  scenes, wb.mask, wb.wj0, wb.nwords,
-                                                  wb.pstarts, ps, nbr_stats));
+#define SPHK_DENSITY_LAUNCH(CUT)                                                            \
+    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL((k_density_walk<S, kS, CUT>),               \
+                                                  dim3(grid_for<S>(chunks, kS)),             \
+                                                  dim3(S::threads), 0, st, wb.items,         \
+                                                  wb.nitems, spm, sq, svr, rows, skey,       \
+                                                  cell_start, c, scenes, wb.mask, wb.wj0,    \
+                                                  wb.nwords, wb.pstarts, ps, nbr_stats))
+    if (c.cut) {
+        SPHK_DENSITY_LAUNCH(true);
+    } else {
+        SPHK_DENSITY_LAUNCH(false);
+    }
+#undef SPHK_DENSITY_LAUNCH
     if (walk_shape_split(shape))
         hipLaunchKernelGGL(k_density_finish, dim3((n + 255) / 256), dim3(256), 0, st, spm, svr,
                            rows, cell_start, c, ps, nbr_stats);
@@ -790,15 +812,20 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
     PartSums ps;
     if (hipError_t err = part_sums(wb, n, shape, &ps)) return err;
     const uint32_t chunks = max_chunks(n, c);
-    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL((k_force_walk<S, kS>),
-                                                  dim3(grid_for<S>(chunks, kS)),
-                                                  dim3(S::threads), 0, st, wb.items, wb.nitems,
-                                                  spm, sq, svr, rows, skey, sid,
-                                                  cell_start, c,
-                                                  scenes,
-                                                  wb.mask, wb.wj0, wb.nwords, wb.pstarts, ps,
-                                                  dt_dev, dt, pm, vel,
-                                                  acc, rho, prs));
+#define SPHK_FORCE_LAUNCH(CUT)                                                              \
+    SPHK_DISPATCH_SHAPE(shape, hipLaunchKernelGGL((k_force_walk<S, kS, CUT>),                 \
+                                                  dim3(grid_for<S>(chunks, kS)),             \
+                                                  dim3(S::threads), 0, st, wb.items,         \
+                                                  wb.nitems, spm, sq, svr, rows, skey, sid,  \
+                                                  cell_start, c, scenes, wb.mask, wb.wj0,    \
+                                                  wb.nwords, wb.pstarts, ps, dt_dev, dt, pm, \
+                                                  vel, acc, rho, prs))
+    if (c.cut) {
+        SPHK_FORCE_LAUNCH(true);
+    } else {
+        SPHK_FORCE_LAUNCH(false);
+    }
+#undef SPHK_FORCE_LAUNCH
     if (walk_shape_split(shape))
         hipLaunchKernelGGL(k_force_finish, dim3((n + 255) / 256), dim3(256), 0, st, spm, svr,
                            sid, cell_start, c, scenes, ps, dt_dev, dt, pm, vel, acc, rho, prs);

@@ -585,3 +585,120 @@ def test_batched_scenes_against_oracle(sphk, oracle, scenes):
         a, b = int(offs[i]), int(offs[i + 1])
         seg = idx_n[offs_n[a]:offs_n[b]]
         assert seg.min() >= a and seg.max() < b
+
+
+# ---------------------------------------------------------------------------
+# physics regimes the step-0 scenes above leave open
+# ---------------------------------------------------------------------------
+def _shape_index(sphk, name):
+    return sphk.walk_shape_names().index(name)
+
+
+def _params_pair(sphk, oracle, scene, formulation=0, **kw):
+    p = scene.params(sphk, formulation)
+    op = scene.oracle_params(oracle, formulation)
+    for k, v in kw.items():
+        setattr(p, k, v)
+        setattr(op, k, v)
+    return p, op
+
+
+def _one_step_vs_oracle(sphk, oracle, scene, p, op, shape=None, dt=None):
+    """One GPU step and the oracle's phases on the same state: neighbour lists, density and
+    acceleration errors (FP32 oracle, FP64 over the same lists) and the conditioning scale."""
+    e = sphk.Engine(scene.n, p)
+    if shape is not None:
+        v = _shape_index(sphk, shape)
+        e.set_walk_shapes(v, v)
+    e.upload(scene.pos, scene.vel)
+    o = oracle.Oracle(scene.n, op)
+    o.set_particles(scene.pos, scene.vel)
+    _oracle_phases(o)
+    offs, idx = _check_neighbors(e, o)
+    e.step(dt or scene.dt)
+    n = scene.n
+    rho_g = e.densities()[:n]
+    rho_o = o.get(2)
+    mass = np.full(n, scene.mass, np.float32)
+    rho64, acc64 = oracle.density_accel_f64(scene.pos, scene.vel, mass, offs, idx, op)
+    scale = oracle.accel_term_scale(scene.pos, scene.vel, mass, rho_o, offs, idx, op)
+    acc_g = e.accelerations().astype(np.float64)
+    return dict(
+        rho_rel=np.abs(rho_g - rho_o) / np.abs(rho_o),
+        rho_rel64=np.abs(rho_g - rho64) / np.abs(rho64),
+        acc_err=np.linalg.norm(acc_g - o.get(4), axis=1) / scale,
+        acc_err64=np.linalg.norm(acc_g - acc64, axis=1) / scale,
+        acc_g=acc_g, acc_o=o.get(4).astype(np.float64), scale=scale)
+
+
+@pytest.mark.parametrize("h", [0.025, 0.015])
+@pytest.mark.parametrize("shape", ["seg32_waves4", "seg32_waves4_split"])
+def test_radius_differs_from_support(sphk, oracle, scenes, h, shape):
+    # The reference keeps its 0.04 m search radius when h changes (examples/dam_break.cpp
+    # sets h = 0.025 and leaves neighbor_search_radius at its default).  With R < 2h pairs
+    # just beyond R would still carry kernel weight: they must not be counted.  R > 2h: pairs
+    # between 2h and R are neighbours of zero weight.
+    s = scenes.config0(seed=17)
+    p, op = _params_pair(sphk, oracle, s, smoothing_length=h, neighbor_search_radius=0.04)
+    r = _one_step_vs_oracle(sphk, oracle, s, p, op, shape)
+    print(f"h={h} {shape}: rho rel {r['rho_rel'].max():.2e} (fp64 {r['rho_rel64'].max():.2e}), "
+          f"accel {r['acc_err'].max():.2e} (fp64 {r['acc_err64'].max():.2e})")
+    assert r["rho_rel"].max() <= RHO_TOL
+    assert r["rho_rel64"].max() <= RHO_TOL
+    assert r["acc_err"].max() <= ACC_TOL
+    assert r["acc_err64"].max() <= ACC_TOL
+
+
+def _moving(scene, seed, sigma=0.3):
+    rng = np.random.default_rng(seed)
+    scene.vel = (rng.normal(size=scene.vel.shape) * sigma).astype(np.float32)
+    return scene
+
+
+@pytest.mark.parametrize("formulation", [0, 1])
+@pytest.mark.parametrize("shape", ["seg32_waves4", "seg32_waves4_split"])
+def test_viscosity_term_isolated(sphk, oracle, scenes, formulation, shape):
+    # gas_constant = 0 and gravity = 0: the acceleration is the viscosity sum alone
+    # (compute_viscosity_force, src/sph_engine.cpp:414-432, with laplacianW_3d,
+    # src/kernels.cpp:130-138); with seeded velocities N(0, 0.3) it is of order 1e4 m/s^2.
+    # The scale S_i is then the sum of the viscosity terms' magnitudes alone.
+    s = _moving(scenes.config0(seed=23), seed=5)
+    p, op = _params_pair(sphk, oracle, s, formulation, gas_constant=0.0, gravity=0.0)
+    r = _one_step_vs_oracle(sphk, oracle, s, p, op, shape)
+    print(f"f={formulation} {shape}: viscosity accel err {r['acc_err'].max():.2e} "
+          f"(fp64 {r['acc_err64'].max():.2e}), median |a| {np.median(np.linalg.norm(r['acc_o'], axis=1)):.3e}")
+    assert r["acc_err"].max() <= ACC_TOL
+    assert r["acc_err64"].max() <= ACC_TOL
+    # the term is far from negligible: a sign or factor error would be of order S_i
+    mag = np.linalg.norm(r["acc_o"], axis=1) / r["scale"]
+    assert np.median(mag) > 1e-2
+
+
+@pytest.mark.parametrize("shape", ["seg32_waves4", "seg32_waves4_split"])
+def test_mixed_pressure_viscosity_moving_state(sphk, oracle, scenes, shape):
+    # Reference formulation from a moving state, with a gas constant small enough that the
+    # pressure and viscosity sums are of the same size: an error in either shows in S_i.
+    s = _moving(scenes.config0(seed=29), seed=6)
+    # |P| ~ 2e-4 * 333 Pa: about half of S_i is the viscosity sum (0.48 on this scene)
+    p, op = _params_pair(sphk, oracle, s, 0, gas_constant=2e-4)
+    r = _one_step_vs_oracle(sphk, oracle, s, p, op, shape)
+    # the viscosity share of the scale, from the same run with k = 0
+    p0, op0 = _params_pair(sphk, oracle, s, 0, gas_constant=0.0)
+    r0 = _one_step_vs_oracle(sphk, oracle, s, p0, op0, shape)
+    share = np.median((r0["scale"] - 9.81) / r["scale"])
+    print(f"{shape}: accel err {r['acc_err'].max():.2e} (fp64 {r['acc_err64'].max():.2e}); "
+          f"viscosity share of S_i {share:.2f}")
+    assert 0.2 < share < 0.8
+    assert r["acc_err"].max() <= ACC_TOL
+    assert r["acc_err64"].max() <= ACC_TOL
+
+
+def test_reference_step_from_moving_state(sphk, oracle, scenes):
+    # the full reference parameters (k = 2000) from a moving state: pressure dominates S_i,
+    # but the walks see nonzero neighbour velocities in every pair
+    s = _moving(scenes.config0(seed=31), seed=7)
+    p, op = _params_pair(sphk, oracle, s, 0)
+    r = _one_step_vs_oracle(sphk, oracle, s, p, op, None)
+    assert r["rho_rel"].max() <= RHO_TOL
+    assert r["acc_err"].max() <= ACC_TOL
+    assert r["acc_err64"].max() <= ACC_TOL

@@ -337,3 +337,97 @@ def test_total_mass_definition(oracle):
     o.step(0.001)
     rho = o.get(oracle.OR_DENSITY)
     assert o.lib.or_total_mass(o.h) == pytest.approx(rho[0] * 0.02 ** 3, rel=1e-6)
+
+
+def test_total_energy_closed_forms(oracle):
+    # get_total_energy = sum 0.5 m |v|^2 (src/sph_engine.cpp:193-200).  Values chosen so that
+    # every product and partial sum is exact in FP32, so the closed forms hold bit for bit.
+    n = 64
+    pos = (np.arange(3 * n, dtype=np.float32).reshape(n, 3) * np.float32(0.1))
+    o = oracle.Oracle(n)
+    # uniform velocity: N * 0.5 m |v|^2 = 64 * 0.5 * 0.5 * (0.25 + 0.0625) = 5.0
+    vel = np.tile(np.array([0.5, 0.25, 0.0], np.float32), (n, 1))
+    o.set_particles(pos, vel, np.full(n, 0.5, np.float32))
+    assert o.lib.or_total_energy(o.h) == 5.0
+    # two speeds, two masses: 32 * 0.5 * 2 * 1 + 32 * 0.5 * 0.25 * 4 = 32 + 16
+    vel = np.zeros((n, 3), np.float32)
+    vel[::2, 0] = 1.0
+    vel[1::2, 1] = -2.0
+    mass = np.where(np.arange(n) % 2 == 0, 2.0, 0.25).astype(np.float32)
+    o.set_particles(pos, vel, mass)
+    assert o.lib.or_total_energy(o.h) == 48.0
+    # all three components enter: |(1, 2, 2)|^2 = 9
+    vel = np.tile(np.array([1.0, 2.0, -2.0], np.float32), (n, 1))
+    o.set_particles(pos, vel, np.full(n, 1.0, np.float32))
+    assert o.lib.or_total_energy(o.h) == 0.5 * 9 * n
+    # random state against FP64 (sequential FP32 sum: error below n * 2^-23 relative)
+    rng = np.random.default_rng(4)
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    mass = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    o.set_particles(pos, vel, mass)
+    ke64 = float(np.sum(0.5 * mass.astype(np.float64) * np.sum(vel.astype(np.float64) ** 2, 1)))
+    assert o.lib.or_total_energy(o.h) == pytest.approx(ke64, rel=n * 2.0 ** -23)
+    # the particles at rest add nothing
+    o.set_particles(pos, np.zeros((n, 3), np.float32))
+    assert o.lib.or_total_energy(o.h) == 0.0
+
+
+def test_radius_below_support_drops_far_pairs(oracle):
+    # h = 0.025 with the default radius 0.04 (examples/dam_break.cpp): a pair 0.045 apart has
+    # q = 1.8 < 2 but lies beyond the radius, so it is not a neighbour and adds nothing; a
+    # pair 0.035 apart is one.
+    h = 0.025
+    sig = np.float32(1.0) / (np.float32(math.pi) * np.float32(h) ** 3)
+    params = oracle.default_params(smoothing_length=h, neighbor_search_radius=0.04,
+                                   gravity=0.0)
+    pos = np.array([[0.1, 0.5, 0.5], [0.145, 0.5, 0.5], [0.5, 0.5, 0.5], [0.535, 0.5, 0.5]],
+                   np.float32)
+    o = oracle.Oracle(4, params)
+    o.set_particles(pos)
+    o.lib.or_update_neighbor_lists(o.h)
+    offs, idx = o.neighbors()
+    assert idx[offs[0]:offs[1]].tolist() == [0]
+    assert sorted(idx[offs[2]:offs[3]].tolist()) == [2, 3]
+    o.lib.or_compute_densities(o.h)
+    rho = o.get(oracle.OR_DENSITY)
+    w0 = float(sig) * 2.0 / 3.0
+    assert rho[0] == pytest.approx(0.001 * w0, rel=1e-6)
+    q = 0.035 / h
+    w = float(sig) * (2.0 - q) ** 3 / 6.0
+    assert rho[2] == pytest.approx(0.001 * (w0 + w), rel=1e-5)
+
+
+def test_sampled_step0_equals_full_phases(oracle, scenes):
+    # The sampled evaluation (for scenes too large for per-particle lists) must give exactly
+    # the full phases' values: same hash, same query, same per-particle loops.
+    s = scenes.config0(seed=13)
+    s.vel = np.random.default_rng(2).normal(0, 0.3, s.pos.shape).astype(np.float32)
+    params = s.oracle_params(oracle)
+    full = oracle.Oracle(s.n, params)
+    full.set_particles(s.pos, s.vel)
+    full.lib.or_update_neighbor_lists(full.h)
+    full.lib.or_compute_densities(full.h)
+    full.lib.or_compute_pressures(full.h)
+    full.lib.or_compute_forces(full.h)
+    offs, idx = full.neighbors()
+    mass = np.full(s.n, s.mass, np.float32)
+    rho64, acc64 = oracle.density_accel_f64(s.pos, s.vel, mass, offs, idx, params)
+    scale = oracle.accel_term_scale(s.pos, s.vel, mass, full.get(2), offs, idx, params)
+    sample = np.random.default_rng(1).choice(s.n, 300, replace=False)
+    sample = np.concatenate([sample, sample[:5]])  # repeated entries are allowed
+    o = oracle.Oracle(s.n, params)
+    o.set_particles(s.pos, s.vel)
+    r = o.sample_step0(sample)
+    np.testing.assert_array_equal(r["counts"], np.diff(offs)[sample])
+    np.testing.assert_array_equal(r["rho"], full.get(2)[sample])
+    np.testing.assert_array_equal(r["acc"], full.get(4)[sample])
+    np.testing.assert_array_equal(r["rho64"], rho64[sample])
+    np.testing.assert_array_equal(r["acc64"], acc64[sample])
+    np.testing.assert_array_equal(r["scale"], scale[sample])
+    so, si = o.sample_neighbors(sample)
+    for k, i in enumerate(sample):
+        assert si[so[k]:so[k + 1]].tolist() == idx[offs[i]:offs[i + 1]].tolist()
+    tot, mx = o.count_neighbors()
+    assert tot == int(offs[-1]) and mx == int(np.diff(offs).max())
+    with pytest.raises(ValueError):
+        o.sample_step0([s.n])
