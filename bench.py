@@ -196,9 +196,12 @@ def ensemble_line(sphk, sph_scenes, scale, steps):
 def full_run_line(sphk, scene, steps, dt=1e-4, every=100, damping=None):
     """`steps` consecutive steps of the 1M dam break under the consistent formulation (the
     reference formulas pile the block into the box corners within a step), timed in segments
-    of `every` steps; between segments (untimed) the kinetic energy (get_total_energy), the
-    potential energy in the gravity field and the reference's conservation check
-    (compute_conservation_errors: |sum rho h^3 - N m| / N m) are read back."""
+    of `every` steps.  Between segments (untimed) the energies are read back: kinetic
+    (get_total_energy), potential in the gravity field (which acts on y), and the internal
+    energy of the equation of state P = k (rho - rho0): U = sum m k (ln(rho/rho0) + rho0/rho - 1),
+    from de = P / rho^2 drho with e(rho0) = 0, at the summation density of the state.  Beside
+    them, the reference's conservation check (compute_conservation_errors: |sum rho h^3 - N m|
+    / N m, energy_error fixed at 0)."""
     import numpy as np
 
     params = scene.params(sphk, formulation=1)
@@ -208,35 +211,39 @@ def full_run_line(sphk, scene, steps, dt=1e-4, every=100, damping=None):
     e.upload(scene.pos, scene.vel)
     p = e.params()
     m = e.masses().astype(np.float64)
-    g = -float(p.gravity)  # gravity acts on y (compute_external_forces)
+    g = -float(p.gravity)
+    k, rho0 = float(p.gas_constant), float(p.rest_density)
 
     def energies():
         y = e.positions()[:, 1].astype(np.float64)
+        rho = e.surrogate_densities().astype(np.float64)  # summation density of this state
         ke = float(e.total_energy())
         pe = float(np.sum(m * g * (y - float(p.ymin))))
-        return ke, pe
+        u = float(np.sum(m * k * (np.log(rho / rho0) + rho0 / rho - 1.0)))
+        return ke, pe, u, rho
 
     traj = []
-    ke, pe = energies()
-    traj.append({"step": 0, "ke": ke, "pe": pe, "e": ke + pe, "mass_error": None})
     t_steps = 0.0
     done = 0
-    while done < steps:
-        k = min(every, steps - done)
+    while True:
+        ke, pe, u, rho = energies()
+        row = {"step": done, "ke": ke, "pe": pe, "u": u, "e": ke + pe + u,
+               "rho_mean": float(rho.mean()),
+               "rho_max_dev_frac": float(np.abs(rho - rho0).max() / rho0)}
+        if done:
+            me, ee = e.conservation_errors()
+            row.update(mass_error=float(me), energy_error=float(ee))
+        traj.append(row)
+        if done >= steps:
+            break
+        kk = min(every, steps - done)
         e.synchronize()
         t0 = time.perf_counter()
-        for _ in range(k):
+        for _ in range(kk):
             e.step(dt)
         e.synchronize()
         t_steps += time.perf_counter() - t0
-        done += k
-        ke, pe = energies()
-        me, ee = e.conservation_errors()
-        rho = e.densities()[: scene.n].astype(np.float64)
-        traj.append({"step": done, "ke": ke, "pe": pe, "e": ke + pe, "mass_error": float(me),
-                     "energy_error": float(ee), "rho_mean": float(rho.mean()),
-                     "rho_max_dev_frac": float(np.abs(rho - p.rest_density).max() /
-                                               p.rest_density)})
+        done += kk
     e0, e1 = traj[0]["e"], traj[-1]["e"]
     out = {
         "workload": f"{scene.name}: {steps} consecutive steps, consistent formulation, dt={dt}, "
@@ -244,8 +251,7 @@ def full_run_line(sphk, scene, steps, dt=1e-4, every=100, damping=None):
         "steps": steps, "value": scene.n * steps / t_steps, "unit": "particle-updates/s",
         "ms_per_step": t_steps / steps * 1e3,
         "energy_drift_frac": (e1 - e0) / e0,
-        "ke_final": traj[-1]["ke"],
-        "mass_exact": float(m.sum()) == float(np.sum(e.masses().astype(np.float64))),
+        "mass_exact": bool(np.array_equal(e.masses().astype(np.float64), m)),
         "reference_conservation_check": {"mass_error_final": traj[-1]["mass_error"],
                                          "energy_error_final": traj[-1]["energy_error"],
                                          "note": "the reference's check: |sum rho h^3 - N m| "
@@ -271,11 +277,18 @@ def graph_line(sphk, sph_scenes, steps):
     return res
 
 
-def surrogate_line(sphk, sph_scenes, snapshots, macro_steps):
+def surrogate_line(sphk, sph_scenes, snapshots, macro_steps, judge_every=4):
     """Learned macro-step surrogate on the 1M dam break (consistent formulation, dt = 1e-4):
     trained on the engine's own trajectory, then timed against plain SPH over the same
     simulated time from the trained-to state.  Density errors are mean |rho - rho_SPH| in
-    kg/m^3 (rest density 1000)."""
+    kg/m^3 (rest density 1000).
+
+    The headline surrogate figure is the main run: judge runs every `judge_every`
+    macro-steps measure the network's divergence per macro-step e and bound the run's density
+    error against the plain run by (4/3) e (N - 1)^(3/2); the network runs only while that bound
+    is within 1 % of the rest density, so the error is held by construction (and measured
+    here).  The network alone (no judge, no bound) is timed beside it as a diagnostic, with its
+    error against the plain run."""
     import numpy as np
 
     scene = sph_scenes.config1()
@@ -291,6 +304,10 @@ def surrogate_line(sphk, sph_scenes, snapshots, macro_steps):
                               iters_per_snapshot=1, lr=1e-3, seed=1)
     e.synchronize()
     t_train = time.perf_counter() - t0
+    trained = e.surrogate_params()
+    e.surrogate_init(seed=0, config=sphk.surrogate_default_config(judge_every=judge_every))
+    e.surrogate_set_params(trained)
+    cfg = e.surrogate_config()
     e.snapshot_save()
     # warm both paths (first-use allocations), then restore; the plain step also counts the
     # neighbour lists of the trained-to state (edges of the network's layer 1)
@@ -314,45 +331,51 @@ def surrogate_line(sphk, sph_scenes, snapshots, macro_steps):
     e.synchronize()
     t_sur = time.perf_counter() - t0
     rho_sur_run = e.surrogate_densities()
-    # 16 macro-steps without a judge against 64 plain steps (untimed)
-    e.snapshot_restore()
-    for _ in range(sub * 16):
-        e.step(dt)
-    rho_plain16 = e.surrogate_densities()
+    # the network alone over the same simulated time (no judge): timed, and its error
     e.snapshot_restore()
     e.synchronize()
     t0 = time.perf_counter()
-    for _ in range(16):
+    for _ in range(macro_steps):
         e.surrogate_macro_step(dt)
     e.synchronize()
-    t_macro = (time.perf_counter() - t0) / 16
-    rho_net16 = e.surrogate_densities()
-    d16 = np.abs(rho_net16 - rho_plain16)
+    t_net = time.perf_counter() - t0
+    rho_net_run = e.surrogate_densities()
     drun = np.abs(rho_sur_run - rho_plain_run)
+    dnet = np.abs(rho_net_run - rho_plain_run)
     ups_plain = n * sub * macro_steps / t_plain
     ups_sur = n * sub * macro_steps / t_sur
+    ups_net = n * sub * macro_steps / t_net
+    t_macro = t_net / macro_steps
     out = {
         "workload": f"dam_break_1M, consistent formulation, dt={dt}, macro-step = {sub} steps, "
-                    f"judge every {cfg.judge_every} macro-steps, fallback above "
-                    f"{cfg.tolerance} kg/m^3 mean density error",
+                    f"judge every {cfg.judge_every} macro-steps, network only while the "
+                    f"bound (4/3) e (N-1)^1.5 on the run's density error stays within "
+                    f"{cfg.tolerance} kg/m^3",
         "value": ups_sur,
         "unit": "particle-updates/s",
         "plain_value": ups_plain,
         "speedup_vs_plain": ups_sur / ups_plain,
-        "ms_per_macro_step": t_macro * 1e3,
+        "rho_error_vs_plain_run": {"mean": float(drun.mean()),
+                                   "mean_frac_rest": float(drun.mean()) / 1000.0,
+                                   "p99": float(np.quantile(drun, 0.99))},
         "ms_per_plain_step": t_plain / (sub * macro_steps) * 1e3,
         "macro_steps": macro_steps,
         "judged_run": {"network_macro_steps": int(rep.macro_steps),
                        "plain_macro_steps": int(rep.plain_macro_steps),
                        "judges": int(rep.judges), "fell_back": bool(rep.fell_back),
-                       "judge_mean_rho_error": rep.max_error,
+                       "fallback_at": int(rep.fallback_at) if rep.fell_back else None,
+                       "error_bound": rep.error_bound,
+                       "divergence_allowing_the_network": cfg.tolerance / (
+                           4.0 / 3.0 * (macro_steps - 1) ** 1.5),
+                       "judge_mean_rho_error_max": rep.max_error,
                        "judge_max_particle_rho_error": rep.max_abs_error},
-        "rho_error_vs_plain_run": {"mean": float(drun.mean()),
-                                   "mean_frac_rest": float(drun.mean()) / 1000.0,
-                                   "p99": float(np.quantile(drun, 0.99))},
-        "rho_error_16_unjudged_macro_steps": {"mean": float(d16.mean()),
-                                              "mean_frac_rest": float(d16.mean()) / 1000.0,
-                                              "p99": float(np.quantile(d16, 0.99))},
+        "network_only": {"value": ups_net, "speedup_vs_plain": ups_net / ups_plain,
+                         "ms_per_macro_step": t_macro * 1e3,
+                         "rho_error_vs_plain_run": {
+                             "mean": float(dnet.mean()),
+                             "mean_frac_rest": float(dnet.mean()) / 1000.0,
+                             "p99": float(np.quantile(dnet, 0.99))},
+                         "note": "no judge and no bound: a diagnostic, not the surrogate figure"},
         "training": {"snapshots": snapshots, "batch_particles": 1024 * 64, "seconds": t_train,
                      "loss_first": float(curve[0]), "loss_last": float(curve[-20:].mean())},
         "dtype": "bf16 MFMA operands, f32 accumulation",
@@ -369,6 +392,24 @@ def surrogate_line(sphk, sph_scenes, snapshots, macro_steps):
     return out
 
 
+def hip_copy_calibration(mib):
+    """Three device-to-device copies of `mib` MiB through the HIP runtime libsphk uses: the
+    reference launches whose FETCH_SIZE / WRITE_SIZE counters scale the walks' counters to
+    bytes."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so.7")
+    nbytes = C.c_size_t(mib << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), nbytes) == 0 and hip.hipMalloc(C.byref(dst), nbytes) == 0
+    assert hip.hipMemset(src, 0, nbytes) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, nbytes, 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -379,13 +420,6 @@ def main():
 
     import sph_scenes
     import sphk
-
-    try:
-        import torch
-
-        sync_dev = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only
-        sync_dev = lambda: None  # noqa: E731
 
     scene = sph_scenes.CONFIGS[args.config]()
     e = sphk.Engine(scene.n, scene.params(sphk))
@@ -407,16 +441,13 @@ def main():
     for _ in range(args.warmup):
         one_step()
     e.synchronize()
-    sync_dev()
     e.reset_stats()
     e.set_timing(True)
     e.synchronize()
-    sync_dev()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         one_step()
     e.synchronize()
-    sync_dev()
     elapsed = time.perf_counter() - t0
     st = e.stats()
     e.set_timing(False)
@@ -498,13 +529,7 @@ def main():
         out["surrogate"] = surrogate_line(sphk, sph_scenes, args.surrogate_snapshots,
                                           args.surrogate_macro)
     if args.calibrate_copy:
-        import torch
-
-        src = torch.ones(args.calibrate_copy * 262144, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+        hip_copy_calibration(args.calibrate_copy)
     if not args.no_cpu_baseline:
         del e
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_repeats, sph_scenes,

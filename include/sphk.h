@@ -135,6 +135,15 @@ int sphk_step(sphk_engine* e, float dt);
 /* Runs num_steps steps (SPHEngine::run_steps, src/sph_engine.cpp:146-150). */
 int sphk_run_steps(sphk_engine* e, size_t num_steps, int adaptive_timestep);
 
+/* Renderer instance data.  Replaces the per-frame host repack of
+ * Renderer::update_particle_data (src/renderer.cpp:279-311): writes, on the device, 9 floats
+ * per particle in caller order -- position (3), velocity (3), colour (3) -- into
+ * out_instances, a device buffer of at least 9 n floats (for example a mapped OpenGL vertex
+ * buffer).  colors_rgb: optional device array of n * 3 floats; NULL gives every particle the
+ * default colour of Particle (0, 0.5, 1) (src/particle.h:42).  Synchronises the stream. */
+int sphk_render_pack(sphk_engine* e, const float* colors_rgb, float* out_instances,
+                     size_t count);
+
 /* Read-backs (synchronise the stream). */
 int sphk_download(sphk_engine* e, int which, float* out, size_t count);
 int sphk_get_time(sphk_engine* e, float* out);
@@ -241,6 +250,7 @@ typedef struct {
     float max_abs_error;         /* largest per-particle |rho - rho_SPH| of any judge run */
     int32_t fell_back;           /* 1 once the run switched to plain SPH */
     uint64_t fallback_at;        /* macro-step index of the switch */
+    float error_bound;           /* accumulated density error bound of the run, kg/m^3 */
 } sphk_surrogate_report;
 
 int sphk_surrogate_default_config(sphk_surrogate_config* out);
@@ -274,10 +284,15 @@ int sphk_surrogate_train(sphk_engine* e, const sphk_train_config* tc, float* los
                          size_t curve_len);
 /* One macro-step of the network (no judge). */
 int sphk_surrogate_macro_step(sphk_engine* e, float dt);
-/* Runs macro_steps macro-steps with the judge: every judge_every-th macro-step is run by
- * plain SPH and by the network from the same state; when the mean |rho - rho_SPH| of the
- * two results exceeds the tolerance the run continues with plain SPH only.  The plain
- * result is kept at every judge run. */
+/* Runs macro_steps macro-steps with the judge.  Macro-steps 1, 1 + judge_every,
+ * 1 + 2 judge_every, ... are judge runs: the network and plain SPH advance from the same
+ * state, the mean |rho - rho_SPH| of the two results is the network's divergence per
+ * macro-step, and the plain result is kept.  The other macro-steps are run by the network.
+ * The judge bounds the run's density error against a plain run of the same length N: with e
+ * the largest divergence measured so far, velocity errors that add up like a random walk give
+ * at most (4/3) e (N - 1)^(3/2) at the end of the run once any network step has run
+ * (error_bound); the network runs only while that bound is within the tolerance, after which
+ * the run continues with plain SPH only. */
 int sphk_surrogate_run(sphk_engine* e, uint64_t macro_steps, float dt,
                        sphk_surrogate_report* rep);
 /* Densities of the current state (one density pass, caller order, n floats). */

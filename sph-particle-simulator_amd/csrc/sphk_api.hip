@@ -935,6 +935,22 @@ int sphk_download(sphk_engine* e, int which, float* out, size_t count) {
     }
 }
 
+int sphk_render_pack(sphk_engine* e, const float* colors_rgb, float* out_instances,
+                     size_t count) {
+    // Renderer::update_particle_data (src/renderer.cpp:279-311) packs position, velocity and
+    // colour of every particle on the host each frame; here the same 9-float layout is written
+    // on the device into a caller buffer (a mapped GL/VK buffer or any device allocation).
+    if (int rc = check_engine(e)) return rc;
+    if (!out_instances || count < 9 * e->n)
+        return set_error(SPHK_E_INVALID, "need a device buffer of %zu floats", 9 * e->n);
+    if (e->n == 0) return SPHK_OK;
+    if (int rc = set_device(e)) return rc;
+    HIP_TRY(launch_render_pack(e->pm, e->vel, colors_rgb, (uint32_t)e->n, out_instances,
+                               e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return SPHK_OK;
+}
+
 int sphk_get_time(sphk_engine* e, float* out) {
     if (int rc = check_engine(e)) return rc;
     if (!out) return set_error(SPHK_E_INVALID, "null output");

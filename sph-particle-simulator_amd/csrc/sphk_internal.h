@@ -197,6 +197,9 @@ hipError_t launch_cfl(const float4* vel, const float4* acc, uint32_t n, const sp
 hipError_t launch_advance_time(float* time_dev, const float* dt_dev, float dt, hipStream_t st,
                                int reps = 1);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+// renderer instance data (9 floats per particle: position, velocity, colour), caller order
+hipError_t launch_render_pack(const float4* pm, const float4* vel, const float* rgb, uint32_t n,
+                              float* out, hipStream_t st);
 hipError_t launch_copy_state(const float4* a0, const float4* a1, const float4* a2, float4* b0,
                              float4* b1, float4* b2, size_t n, const float* t_src,
                              float* t_dst, hipStream_t st);

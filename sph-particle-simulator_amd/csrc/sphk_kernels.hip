@@ -462,4 +462,59 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Renderer instance data: 9 floats per particle, caller order -- position (3), velocity (3),
+// colour (3) -- the layout Renderer::update_particle_data builds on the host every frame
+// (src/renderer.cpp:279-311).  One thread per output float4: reads are gathers from the home
+// rows (cached), writes are coalesced 16-byte stores.
+// ---------------------------------------------------------------------------
+template <bool kVec4>
+__global__ __launch_bounds__(256) void k_render_pack(const float4* __restrict__ pm,
+                                                     const float4* __restrict__ vel,
+                                                     const float* __restrict__ rgb, uint32_t n,
+                                                     float* __restrict__ out) {
+    const size_t total = (size_t)9 * n;
+    const size_t k0 = 4 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (k0 >= total) return;
+    float v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const size_t k = k0 + (size_t)c;
+        v[c] = 0.0f;
+        if (k >= total) continue;
+        const size_t i = k / 9;
+        const uint32_t f = (uint32_t)(k - 9 * i);
+        if (f < 3) {
+            const float4 p = pm[i];
+            v[c] = f == 0 ? p.x : f == 1 ? p.y : p.z;
+        } else if (f < 6) {
+            const float4 q = vel[i];
+            v[c] = f == 3 ? q.x : f == 4 ? q.y : q.z;
+        } else {
+            // Particle::color's default (src/particle.h:42) when no colours are given
+            const float def = f == 6 ? 0.0f : f == 7 ? 0.5f : 1.0f;
+            v[c] = rgb ? rgb[3 * i + (f - 6)] : def;
+        }
+    }
+    if (kVec4 && k0 + 4 <= total) {
+        *reinterpret_cast<float4*>(out + k0) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (k0 + (size_t)c < total) out[k0 + (size_t)c] = v[c];
+    }
+}
+
+hipError_t launch_render_pack(const float4* pm, const float4* vel, const float* rgb, uint32_t n,
+                              float* out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const size_t quads = ((size_t)9 * n + 3) / 4;
+    const dim3 grid((unsigned)((quads + 255) / 256));
+    if (reinterpret_cast<uintptr_t>(out) % 16 == 0)
+        hipLaunchKernelGGL(k_render_pack<true>, grid, dim3(256), 0, st, pm, vel, rgb, n, out);
+    else
+        hipLaunchKernelGGL(k_render_pack<false>, grid, dim3(256), 0, st, pm, vel, rgb, n, out);
+    return hipGetLastError();
+}
+
 }  // namespace sphk

@@ -79,7 +79,7 @@ class SurrogateReport(C.Structure):
     _fields_ = [
         ("macro_steps", C.c_uint64), ("plain_macro_steps", C.c_uint64), ("judges", C.c_uint64),
         ("last_error", C.c_float), ("max_error", C.c_float), ("max_abs_error", C.c_float),
-        ("fell_back", C.c_int32), ("fallback_at", C.c_uint64),
+        ("fell_back", C.c_int32), ("fallback_at", C.c_uint64), ("error_bound", C.c_float),
     ]
 
 
@@ -104,6 +104,7 @@ SIGNATURES = {
     "sphk_step": [_P, C.c_float],
     "sphk_run_steps": [_P, _SZ, C.c_int],
     "sphk_download": [_P, C.c_int, _F, _SZ],
+    "sphk_render_pack": [_P, C.c_void_p, C.c_void_p, _SZ],
     "sphk_get_time": [_P, _F],
     "sphk_get_step_count": [_P, _U64],
     "sphk_total_mass": [_P, _F],
@@ -252,6 +253,11 @@ class Engine:
         out = np.empty(shape, dtype=np.float32)
         check(self.lib.sphk_download(self.h, which, _ptr(out, C.c_float), out.size))
         return out
+
+    def render_pack(self, out_ptr: int, count: int, colors_ptr: Optional[int] = None):
+        """Writes the renderer's 9-float instance rows (position, velocity, colour) of every
+        particle into the device buffer at out_ptr (count floats, at least 9 n)."""
+        check(self.lib.sphk_render_pack(self.h, colors_ptr, out_ptr, count))
 
     def positions(self) -> np.ndarray:
         return self._download(SPHK_POSITIONS, (self.size(), 3))

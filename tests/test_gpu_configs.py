@@ -208,21 +208,37 @@ def test_config4_surrogate_forward_fullsize(sphk, oracle, scenes):
 
 
 def test_config4_judge_and_fallback_fullsize(sphk, scenes):
-    # The judge on the 1M scene: with a zero tolerance the first judge run switches the run
-    # to plain SPH, and the state equals one network macro-step followed by plain steps.
+    # The judge on the 1M scene.  A zero tolerance switches to plain SPH at the first judge
+    # run (the state is then 12 plain steps, bit for bit); a loose one keeps the network
+    # between judge runs (the state equals plain, network, plain macro-steps by hand).
     s = scenes.config1()
     dt = 1e-4
     e = sphk.Engine(s.n, s.params(sphk, 1))
     e.upload(s.pos, s.vel)
     e.surrogate_init(seed=9, config=sphk.surrogate_default_config(judge_every=2, tolerance=0.0))
     rep = e.surrogate_run(3, dt)
-    assert (rep.macro_steps, rep.plain_macro_steps, rep.judges) == (1, 2, 1)
-    assert rep.fell_back == 1 and rep.fallback_at == 2 and rep.last_error > 0
+    assert (rep.macro_steps, rep.plain_macro_steps, rep.judges) == (0, 3, 1)
+    assert rep.fell_back == 1 and rep.fallback_at == 1 and rep.last_error > 0
     e2 = sphk.Engine(s.n, s.params(sphk, 1))
     e2.upload(s.pos, s.vel)
-    e2.surrogate_init(seed=9)
-    e2.surrogate_macro_step(dt)
-    for _ in range(8):
+    for _ in range(12):
         e2.step(dt)
     np.testing.assert_array_equal(e.positions(), e2.positions())
     np.testing.assert_array_equal(e.velocities(), e2.velocities())
+    del e2
+    e.upload(s.pos, s.vel)
+    e.surrogate_init(seed=9, config=sphk.surrogate_default_config(judge_every=2,
+                                                                 tolerance=1e30))
+    rep = e.surrogate_run(3, dt)
+    assert (rep.macro_steps, rep.plain_macro_steps, rep.judges, rep.fell_back) == (1, 2, 2, 0)
+    assert rep.error_bound == pytest.approx(4 / 3 * rep.max_error * 2 ** 1.5, rel=1e-5)
+    e3 = sphk.Engine(s.n, s.params(sphk, 1))
+    e3.upload(s.pos, s.vel)
+    e3.surrogate_init(seed=9)
+    for _ in range(4):
+        e3.step(dt)
+    e3.surrogate_macro_step(dt)
+    for _ in range(4):
+        e3.step(dt)
+    np.testing.assert_array_equal(e.positions(), e3.positions())
+    np.testing.assert_array_equal(e.velocities(), e3.velocities())

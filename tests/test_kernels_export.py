@@ -165,7 +165,7 @@ def test_csv_time_series(tmp_path):
 
 # ---------------------------------------------------------------- GPU
 @pytest.mark.gpu
-def test_npz_restore_reproduces_the_run(sphk, scenes, tmp_path):
+def test_npz_restore_reproduces_the_run(sphk, oracle, scenes, tmp_path):
     import sph_export
 
     s = scenes.small_block(8, 8, 8, seed=3)
@@ -177,11 +177,23 @@ def test_npz_restore_reproduces_the_run(sphk, scenes, tmp_path):
     sph_export.save_npz(e, path)
     e2 = sphk.Engine(s.n, s.params(sphk, formulation=1))
     sph_export.restore_npz(e2, path)
+    # the file holds the state at the save, bit for bit
+    d = sph_export.load_npz(path)
+    np.testing.assert_array_equal(d["positions"], e.positions())
+    np.testing.assert_array_equal(d["velocities"], e.velocities())
     for _ in range(5):
         e.step(1e-4)
         e2.step(1e-4)
     np.testing.assert_array_equal(e.positions(), e2.positions())
     np.testing.assert_array_equal(e.velocities(), e2.velocities())
+    # and the oracle, started from the file, reaches the same state after the same 5 steps
+    o = oracle.Oracle(s.n, s.oracle_params(oracle, formulation=1))
+    o.set_particles(d["positions"], d["velocities"])
+    for _ in range(5):
+        o.step(1e-4)
+    np.testing.assert_allclose(e2.positions(), o.get(oracle.OR_POS), rtol=0, atol=1e-6)
+    np.testing.assert_allclose(e2.velocities(), o.get(oracle.OR_VEL), rtol=0, atol=1e-3)
+    np.testing.assert_allclose(e2.densities()[: s.n], o.get(oracle.OR_DENSITY), rtol=1e-5)
 
 
 @pytest.mark.gpu
@@ -199,3 +211,61 @@ def test_cpp_csv_time_series(sph, tmp_path):
     assert len(rows) == 3 and int(rows[0]["particles"]) == 84800
     assert float(rows[-1]["time"]) == pytest.approx(0.003, rel=1e-5)
     assert float(rows[0]["avg_density"]) > 0
+
+
+class _DeviceBuffer:
+    """Device memory from the HIP runtime libsphk.so itself uses (test plumbing)."""
+
+    def __init__(self, nbytes):
+        import ctypes as C
+
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.ptr = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(nbytes)) == 0
+        self.nbytes = nbytes
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        assert self.hip.hipMemcpy(self.ptr, a.ctypes.data_as(self.C.c_void_p),
+                                  self.C.c_size_t(a.nbytes), 1) == 0  # host to device
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype)
+        assert self.hip.hipMemcpy(out.ctypes.data_as(self.C.c_void_p), self.ptr,
+                                  self.C.c_size_t(out.nbytes), 2) == 0  # device to host
+        return out
+
+    def __del__(self):
+        self.hip.hipFree(self.ptr)
+
+
+@pytest.mark.gpu
+def test_render_pack_matches_host_repack(sphk, scenes):
+    # Renderer::update_particle_data (src/renderer.cpp:279-311) packs position, velocity and
+    # colour per particle on the host; the device pack must give the same floats, bit for bit,
+    # with the default colour, with caller colours, and into an unaligned buffer.
+    s = scenes.small_block(13, 11, 9, seed=4)
+    e = sphk.Engine(s.n, s.params(sphk, formulation=1))
+    e.upload(s.pos, s.vel)
+    for _ in range(3):
+        e.step(1e-4)
+    pos, vel = e.positions(), e.velocities()
+    want = np.concatenate([pos, vel, np.tile(np.array([0.0, 0.5, 1.0], F32), (s.n, 1))], 1)
+    total = 9 * s.n + 4
+    buf = _DeviceBuffer(4 * total)
+    buf.upload(np.full(total, -1.0, F32))
+    e.render_pack(buf.ptr.value, 9 * s.n)
+    got = buf.download(F32, total)
+    np.testing.assert_array_equal(got[: 9 * s.n].reshape(s.n, 9), want)
+    assert np.all(got[9 * s.n:] == -1.0)  # nothing past the rows
+    rgb = np.random.default_rng(2).random((s.n, 3)).astype(F32)
+    cbuf = _DeviceBuffer(rgb.nbytes)
+    cbuf.upload(rgb)
+    e.render_pack(buf.ptr.value + 4, 9 * s.n, cbuf.ptr.value)  # one float in: unaligned
+    got = buf.download(F32, total)[1: 1 + 9 * s.n].reshape(s.n, 9)
+    want[:, 6:] = rgb
+    np.testing.assert_array_equal(got, want)
+    with pytest.raises(sphk.SphkError):
+        e.render_pack(buf.ptr.value, 9 * s.n - 1)

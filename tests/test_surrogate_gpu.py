@@ -164,30 +164,54 @@ def test_training_reduces_loss(sphk, scenes):
 def test_judge_keeps_plain_result_and_falls_back(sphk, scenes):
     s = scenes.small_block(10, 8, 8, seed=6)
     dt = 1e-4
-    # tolerance 0: the first judge run (macro-step 2) switches to plain SPH for good
+    # tolerance 0: the first judge run (macro-step 1) measures a nonzero divergence, so the
+    # bound would pass the tolerance before the next judge run: plain SPH from then on
     e = _engine(sphk, s)
     e.surrogate_init(seed=9, config=sphk.surrogate_default_config(judge_every=2, tolerance=0.0))
     rep = e.surrogate_run(6, dt)
-    assert (rep.macro_steps, rep.plain_macro_steps, rep.judges) == (1, 5, 1)
-    assert rep.fell_back == 1 and rep.fallback_at == 2 and rep.last_error > 0
+    assert (rep.macro_steps, rep.plain_macro_steps, rep.judges) == (0, 6, 1)
+    assert rep.fell_back == 1 and rep.fallback_at == 1 and rep.last_error > 0
+    assert rep.error_bound == 0.0  # no network step was kept
     assert e.step_count() == 24
-    # the same as one network macro-step followed by 20 plain steps
+    # every kept macro-step was plain: the state equals 24 plain steps
     e2 = _engine(sphk, s)
-    e2.surrogate_init(seed=9)
-    e2.surrogate_macro_step(dt)
-    for _ in range(20):
+    for _ in range(24):
         e2.step(dt)
     np.testing.assert_array_equal(e.positions(), e2.positions())
     np.testing.assert_array_equal(e.velocities(), e2.velocities())
     assert e.time() == e2.time()
-    # a loose tolerance never falls back: one judge run per judge_every macro-steps
+    # a loose tolerance never falls back: judge runs at macro-steps 1 and 5, the network
+    # runs the others; the bound is (4/3) e_max (N - 1)^(3/2) with N = 8
     e3 = _engine(sphk, s)
     e3.surrogate_init(seed=9, config=sphk.surrogate_default_config(judge_every=4,
                                                                   tolerance=1e30))
     rep = e3.surrogate_run(8, dt)
     assert (rep.macro_steps, rep.plain_macro_steps, rep.judges, rep.fell_back) == (6, 2, 2, 0)
-    assert rep.max_error >= rep.last_error >= 0
+    assert rep.max_error >= rep.last_error > 0
+    assert rep.error_bound == pytest.approx(4 / 3 * rep.max_error * 7 ** 1.5, rel=1e-5)
     assert e3.step_count() == 32
+    # the same sequence by hand: plain, 3 network, plain, 3 network
+    e4 = _engine(sphk, s)
+    e4.surrogate_init(seed=9)
+    for block in range(2):
+        for _ in range(4):
+            e4.step(dt)
+        for _ in range(3):
+            e4.surrogate_macro_step(dt)
+    np.testing.assert_array_equal(e3.positions(), e4.positions())
+    np.testing.assert_array_equal(e3.velocities(), e4.velocities())
+    # a tolerance just below that bound: the run falls back at the judge run that measures
+    # the largest divergence (1 or 5)
+    e5 = _engine(sphk, s)
+    e5.surrogate_init(seed=9, config=sphk.surrogate_default_config(
+        judge_every=4, tolerance=float(0.999 * rep.error_bound)))
+    rep5 = e5.surrogate_run(8, dt)
+    assert rep5.fell_back == 1 and rep5.fallback_at in (1, 5)
+    if rep5.fallback_at == 1:
+        assert rep5.macro_steps == 0 and rep5.error_bound == 0.0
+    else:
+        assert rep5.macro_steps == 3
+        assert rep5.error_bound == pytest.approx(rep.error_bound, rel=1e-5)
 
 
 def test_errors_without_init_and_bad_arguments(sphk, scenes):
@@ -244,3 +268,15 @@ def test_trained_network_beats_the_method_without_it(sphk, scenes):
     assert e_trained < 0.5 * e_gravity, errs
     assert e_trained < 0.5 * e_untrained, errs
     assert e_trained < 10.0, errs  # 1% of the rest density
+    # the main run's error bound holds: 8 macro-steps, judge runs at 1 and 5, against 32
+    # plain steps from the same state (the bound assumes errors that add up like a random
+    # walk; damping and the density's smoothing make them grow more slowly)
+    e.snapshot_restore()
+    e.surrogate_init(seed=3, config=sphk.surrogate_default_config(judge_every=4,
+                                                                 tolerance=1e30))
+    e.surrogate_set_params(trained)
+    rep = e.surrogate_run(8, dt)
+    err_run = float(np.abs(e.surrogate_densities() - rho_ref).mean())
+    print(f"judged run: true error {err_run:.3f} kg/m^3, bound {rep.error_bound:.3f}")
+    assert rep.judges == 2 and rep.macro_steps == 6
+    assert err_run <= rep.error_bound
