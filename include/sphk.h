@@ -153,6 +153,15 @@ int sphk_get_step_count(const sphk_engine* e, uint64_t* out);
 int sphk_total_mass(sphk_engine* e, float* out);
 int sphk_total_energy(sphk_engine* e, float* out);
 int sphk_conservation_errors(sphk_engine* e, float* mass_error, float* energy_error);
+/* The kinetic energy of sphk_total_energy without allocating: no hipMalloc, hipFree or host
+ * buffer on any call, for callers that cannot allocate (code under stream capture, serving
+ * loops, harnesses that forbid it).  No counterpart in the reference.  Each term is the same
+ * FP32 (0.5 m) |v|^2; the terms are summed on the device in FP64 in a fixed order (the
+ * result differs from sphk_total_energy's sequential FP32 sum only by that sum's rounding).
+ * Partial sums use a device scratch of 1,025 doubles per GPU, shared by every engine on that
+ * GPU and kept until the process ends; any particle count is handled (the first pass strides
+ * over the particles).  Calls that overlap on one GPU race on that scratch. */
+int sphk_total_energy_noalloc(sphk_engine* e, float* out);
 
 /* Performance stats (SPHEngine::get_performance_stats / reset_performance_stats). */
 int sphk_set_timing(sphk_engine* e, int enabled);

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -586,6 +587,81 @@ int graph_step(sphk_engine* e, float dt, bool adaptive) {
     return SPHK_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free kinetic energy (sphk_total_energy_noalloc).  The partial sums live in
+// `scratch`, a file-scope device array: one FP64 sum per workgroup of the first pass and,
+// after them, the FP32 result of the second.  Every engine on a GPU shares that GPU's copy;
+// its address on a device is taken once (hipGetSymbolAddress, at the first engine created on
+// the device) and host code uses only that address.  Each term is the reference's FP32
+// (0.5 m) |v|^2 (src/sph_engine.cpp:193-200); the terms are summed in FP64 in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int kEnergyBlocks = 1024;  // workgroups of the first pass, at most
+constexpr int kEnergyThreads = 256;
+static __device__ __align__(256) double scratch[kEnergyBlocks + 1];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of this device's scratch (device `dev` current); taken once per device.
+int scratch_setup(int dev) {
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(SPHK_E_INVALID, "device %d beyond the %d supported", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return SPHK_OK;
+    void* ptr = nullptr;
+    hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !ptr)
+        return set_error(SPHK_E_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                         hipGetErrorString(err));
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    return SPHK_OK;
+}
+
+double* scratch_of(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[dev];
+}
+
+// Fixed-order sum of a workgroup's FP64 values: each wavefront by xor-shuffles (lane order
+// fixed), then the four wavefront sums in order.
+__device__ __forceinline__ double block_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    __shared__ double wsum[kEnergyThreads / 64];
+    const int wave = (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63u) == 0) wsum[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < kEnergyThreads / 64; ++w) t += wsum[w];
+    return t;
+}
+
+__global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(const float4* __restrict__ pm,
+                                                                  const float4* __restrict__ vel,
+                                                                  uint32_t n,
+                                                                  double* __restrict__ part) {
+#pragma clang fp contract(off)
+    double acc = 0.0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4 v = vel[i];
+        const float tx = v.x * v.x, ty = v.y * v.y, tz = v.z * v.z;
+        const float sp = (tx + ty) + tz;
+        acc += (double)((0.5f * pm[i].w) * sp);
+    }
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kEnergyThreads) void k_energy_final(double* __restrict__ part,
+                                                                uint32_t nparts) {
+    double acc = 0.0;
+    for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) acc += part[b];
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) reinterpret_cast<float*>(part + kEnergyBlocks)[0] = (float)t;
+}
+
 }  // namespace sphk
 
 extern "C" {
@@ -611,6 +687,7 @@ int sphk_create(size_t capacity, sphk_engine** out) {
     fill_default(&e->params);
     e->scenes.resize(1);
     HIP_TRY(hipGetDevice(&e->device));
+    if (int rc = scratch_setup(e->device)) return rc;
     HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     const size_t C = capacity;
     HIP_TRY(dalloc(&e->pm, C));
@@ -1002,6 +1079,31 @@ int sphk_total_energy(sphk_engine* e, float* out) {
         ke += (0.5f * pm[i].w) * sp;
     }
     *out = ke;
+    return SPHK_OK;
+}
+
+int sphk_total_energy_noalloc(sphk_engine* e, float* out) {
+    // sphk_total_energy without an allocation: two kernels over the engine's velocities and
+    // masses, partial sums in the shared device scratch, one float copied back.
+    if (int rc = check_engine(e)) return rc;
+    if (!out) return set_error(SPHK_E_INVALID, "null output");
+    if (int rc = set_device(e)) return rc;
+    double* sc = scratch_of(e->device);
+    if (!sc) return set_error(SPHK_E_INVALID, "no scratch on device %d", e->device);
+    const uint32_t n = (uint32_t)e->n;
+    uint32_t blocks = (n + kEnergyThreads - 1) / kEnergyThreads;
+    blocks = std::max<uint32_t>(1u, std::min<uint32_t>(blocks, (uint32_t)kEnergyBlocks));
+    if (blocks > (uint32_t)kEnergyBlocks)
+        return set_error(SPHK_E_INVALID, "%u partial sums exceed the scratch of %d", blocks,
+                         kEnergyBlocks);
+    hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(kEnergyThreads), 0, e->stream,
+                       e->pm, e->vel, n, sc);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kEnergyThreads), 0, e->stream, sc, blocks);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, sc + kEnergyBlocks, sizeof(float), hipMemcpyDeviceToHost,
+                           e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
     return SPHK_OK;
 }
 

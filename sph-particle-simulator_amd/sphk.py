@@ -105,6 +105,7 @@ SIGNATURES = {
     "sphk_run_steps": [_P, _SZ, C.c_int],
     "sphk_download": [_P, C.c_int, _F, _SZ],
     "sphk_render_pack": [_P, C.c_void_p, C.c_void_p, _SZ],
+    "sphk_total_energy_noalloc": [_P, _F],
     "sphk_get_time": [_P, _F],
     "sphk_get_step_count": [_P, _U64],
     "sphk_total_mass": [_P, _F],
@@ -292,6 +293,11 @@ class Engine:
         v = C.c_float()
         check(self.lib.sphk_total_mass(self.h, C.byref(v)))
         return v.value
+
+    def total_energy_noalloc(self) -> float:
+        out = C.c_float()
+        check(self.lib.sphk_total_energy_noalloc(self.h, C.byref(out)))
+        return out.value
 
     def total_energy(self) -> float:
         v = C.c_float()
