@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device three times "
                          "(reference launches for HBM byte counters)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the headline step from a captured HIP graph")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned macro-step surrogate line")
     ap.add_argument("--surrogate-snapshots", type=int, default=2000,
@@ -433,6 +435,8 @@ def main():
 
         tune = sph_autotune.autotune(e, scene.dt, steps=3, use_cache=False)
     e.snapshot_save()
+    if args.graph:
+        e.set_graph(True)
 
     def one_step():
         e.snapshot_restore()
@@ -442,13 +446,20 @@ def main():
         one_step()
     e.synchronize()
     e.reset_stats()
-    e.set_timing(True)
+    # phase timing records HIP events around each phase and runs the step eagerly; a graph
+    # replay is timed without it, and the phases then come from as many eager steps after
+    e.set_timing(not args.graph)
     e.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         one_step()
     e.synchronize()
     elapsed = time.perf_counter() - t0
+    if args.graph:
+        e.set_timing(True)
+        for _ in range(args.steps):
+            one_step()
+        e.synchronize()
     st = e.stats()
     e.set_timing(False)
 
