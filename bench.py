@@ -195,7 +195,7 @@ def ensemble_line(sphk, sph_scenes, scale, steps):
     return out
 
 
-def full_run_line(sphk, scene, steps, dt=1e-4, every=100, damping=None):
+def full_run_line(sphk, scene, steps, dt=1e-4, every=100, damping=None, shapes=None):
     """`steps` consecutive steps of the 1M dam break under the consistent formulation (the
     reference formulas pile the block into the box corners within a step), timed in segments
     of `every` steps.  Between segments (untimed) the energies are read back: kinetic
@@ -210,6 +210,8 @@ def full_run_line(sphk, scene, steps, dt=1e-4, every=100, damping=None):
     if damping is not None:
         params.damping = damping
     e = sphk.Engine(scene.n, params)
+    if shapes is not None:
+        e.set_walk_shapes(*shapes)
     e.upload(scene.pos, scene.vel)
     p = e.params()
     m = e.masses().astype(np.float64)
@@ -279,7 +281,7 @@ def graph_line(sphk, sph_scenes, steps):
     return res
 
 
-def surrogate_line(sphk, sph_scenes, snapshots, macro_steps, judge_every=4):
+def surrogate_line(sphk, sph_scenes, snapshots, macro_steps, judge_every=4, shapes=None):
     """Learned macro-step surrogate on the 1M dam break (consistent formulation, dt = 1e-4):
     trained on the engine's own trajectory, then timed against plain SPH over the same
     simulated time from the trained-to state.  Density errors are mean |rho - rho_SPH| in
@@ -296,6 +298,8 @@ def surrogate_line(sphk, sph_scenes, snapshots, macro_steps, judge_every=4):
     scene = sph_scenes.config1()
     dt = 1e-4
     e = sphk.Engine(scene.n, scene.params(sphk, formulation=1))
+    if shapes is not None:
+        e.set_walk_shapes(*shapes)
     e.upload(scene.pos, scene.vel)
     e.surrogate_init(seed=0)
     cfg = e.surrogate_config()
@@ -558,12 +562,13 @@ def main():
         out["ensemble"] = ensemble_line(sphk, sph_scenes, args.ensemble_scale, 5)
         out["small_scene_10k"] = graph_line(sphk, sph_scenes, 50)
     if args.full_run:
-        out["full_run"] = full_run_line(sphk, scene, args.full_run)
+        out["full_run"] = full_run_line(sphk, scene, args.full_run, shapes=e.walk_shapes())
         undamped = sph_scenes.CONFIGS[args.config]()
-        out["full_run_undamped"] = full_run_line(sphk, undamped, args.full_run, damping=1.0)
+        out["full_run_undamped"] = full_run_line(sphk, undamped, args.full_run, damping=1.0,
+                                                  shapes=e.walk_shapes())
     if not args.no_surrogate and args.config == 1:
         out["surrogate"] = surrogate_line(sphk, sph_scenes, args.surrogate_snapshots,
-                                          args.surrogate_macro)
+                                          args.surrogate_macro, shapes=e.walk_shapes())
     if args.calibrate_copy:
         hip_copy_calibration(args.calibrate_copy)
     if not args.no_cpu_baseline:

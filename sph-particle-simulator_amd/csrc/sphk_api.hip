@@ -672,6 +672,8 @@ int sphk_default_params(sphk_params* out) {
     return SPHK_OK;
 }
 
+static int create_buffers(sphk_engine* e, size_t capacity);
+
 int sphk_create(size_t capacity, sphk_engine** out) {
     if (!out) return set_error(SPHK_E_INVALID, "null output handle");
     *out = nullptr;
@@ -683,6 +685,16 @@ int sphk_create(size_t capacity, sphk_engine** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_error(SPHK_E_NODEVICE, "no HIP device visible");
     sphk_engine* e = new sphk_engine();
+    if (int rc = create_buffers(e, capacity)) {
+        sphk_destroy(e);  // frees whatever was allocated before the failure
+        return rc;
+    }
+    *out = e;
+    return SPHK_OK;
+}
+
+// Device set-up of a new engine; on failure the caller destroys the partial engine.
+static int create_buffers(sphk_engine* e, size_t capacity) {
     e->cap = capacity;
     fill_default(&e->params);
     e->scenes.resize(1);
@@ -727,7 +739,6 @@ int sphk_create(size_t capacity, sphk_engine** out) {
     HIP_TRY(hipMemsetAsync(e->uscal, 0, 4 * sizeof(uint32_t), e->stream));
     HIP_TRY(hipMemsetAsync(e->nstats, 0, 2 * sizeof(unsigned long long), e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
-    *out = e;
     return SPHK_OK;
 }
 
@@ -1316,6 +1327,33 @@ int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t 
     HIP_TRY(hipStreamSynchronize(e->stream));
     HIP_TRY(hipFree(doffs));
     HIP_TRY(hipFree(didx));
+    return SPHK_OK;
+}
+
+int sphk_debug_walk_words(sphk_engine* e, uint32_t* items, uint32_t* nwords, uint32_t* wj0,
+                          uint32_t* mask, size_t chunk_cap, size_t* nchunks) {
+    if (int rc = check_engine(e)) return rc;
+    if (!nchunks) return set_error(SPHK_E_INVALID, "null chunk count");
+    *nchunks = 0;
+    if (int rc = prepare_sorted(e)) return rc;
+    const uint32_t n = (uint32_t)e->n;
+    if (n == 0) return SPHK_OK;
+    HIP_TRY(launch_density_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->cell_start,
+                                n, e->consts, e->d_scenes, nullptr, e->shape_density, e->stream));
+    uint32_t nc = 0;
+    HIP_TRY(hipMemcpyAsync(&nc, e->walk.nitems, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    *nchunks = nc;
+    if (!items) return SPHK_OK;
+    if (chunk_cap < nc) return set_error(SPHK_E_INVALID, "need %u chunks", nc);
+    if (!nwords || !wj0 || !mask) return set_error(SPHK_E_INVALID, "null output array");
+    const size_t words = (size_t)nc * kMaxWords;
+    HIP_TRY(hipMemcpyAsync(items, e->walk.items, (size_t)nc * 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(nwords, e->walk.nwords, (size_t)nc * 4, hipMemcpyDeviceToHost,
+                           e->stream));
+    HIP_TRY(hipMemcpyAsync(wj0, e->walk.wj0, words * 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(mask, e->walk.mask, words * 64 * 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
     return SPHK_OK;
 }
 
