@@ -540,12 +540,13 @@ int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev) {
     if (int rc = build_sorted(e)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
     HIP_TRY(launch_density_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->cell_start,
-                                n, e->consts, e->d_scenes, e->nstats, e->shape_density, st));
+                                n, e->consts, e->d_scenes, e->shape_density, st));
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
     HIP_TRY(launch_force_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->sid,
                               e->cell_start, n,
                               e->consts, e->d_scenes, adaptive ? e->fscal : nullptr, dt, e->pm,
-                              e->vel, e->acc, e->rho, e->prs, e->shape_force, e->fscal + 1, st));
+                              e->vel, e->acc, e->rho, e->prs, e->shape_force, e->fscal + 1,
+                              e->nstats, st));
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
     return SPHK_OK;
 }
@@ -1339,7 +1340,7 @@ int sphk_debug_walk_words(sphk_engine* e, uint32_t* items, uint32_t* nwords, uin
     const uint32_t n = (uint32_t)e->n;
     if (n == 0) return SPHK_OK;
     HIP_TRY(launch_density_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->cell_start,
-                                n, e->consts, e->d_scenes, nullptr, e->shape_density, e->stream));
+                                n, e->consts, e->d_scenes, e->shape_density, e->stream));
     uint32_t nc = 0;
     HIP_TRY(hipMemcpyAsync(&nc, e->walk.nitems, 4, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));

@@ -175,8 +175,8 @@ hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const u
                                float4* svr,
                                ForceRow* rows, const uint32_t* skey,
                                const uint32_t* cell_start, uint32_t n,
-                               const StepConsts& c, const SceneGrid* scenes,
-                               unsigned long long* nbr_stats, int shape, hipStream_t st);
+                               const StepConsts& c, const SceneGrid* scenes, int shape,
+                               hipStream_t st);
 hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
                              const float4* svr,
                              const ForceRow* rows, const uint32_t* skey,
@@ -184,7 +184,7 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
                              const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
                              float4* vel, float4* acc, float* rho, float* prs, int shape,
-                             float* time_dev, hipStream_t st);
+                             float* time_dev, unsigned long long* nbr_stats, hipStream_t st);
 hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint2* sq,
                             const uint32_t* skey,
                             const uint32_t* sid, const uint32_t* cell_start, uint32_t n,

@@ -199,8 +199,8 @@ int prepare_density(sphk_engine* e) {
     if (int rc = prepare_grid(e)) return rc;
     if (int rc = build_sorted(e)) return rc;
     HIP_TRY(launch_density_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->cell_start,
-                                (uint32_t)e->n, e->consts, e->d_scenes, nullptr,
-                                e->shape_density, e->stream));
+                                (uint32_t)e->n, e->consts, e->d_scenes, e->shape_density,
+                                e->stream));
     return SPHK_OK;
 }
 

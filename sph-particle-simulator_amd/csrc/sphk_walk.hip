@@ -101,30 +101,34 @@ __device__ __forceinline__ float w6_shape8(float r, float half_inv_h, float inv_
     return __builtin_fmaf(s * s, -0.5f * s, u * u * u);
 }
 
-// One neighbour's m_j (6 W / 8 sigma).  |r| comes from the reference's d^2, which the
-// neighbour count (on every step of the engine) needs anyway.  Pairs the filter let through
-// beyond the radius: with a radius of at least 2h they have q >= 2 (up to the rounding of
-// sqrt, which leaves a shape below 2^-75) and weigh nothing; with a smaller radius (kCut) the
-// reference's test d^2 <= r^2 drops them explicitly.
-template <bool kCount, bool kCut>
+// One neighbour's m_j (6 W / 8 sigma).  Pairs the filter let through beyond the radius: with
+// a radius of at least 2h they have q >= 2 (up to the rounding of sqrt, which leaves a shape
+// below 2^-75) and weigh nothing, so d^2 may be formed with fused multiply-adds; with a smaller
+// radius (kCut) the reference's test d^2 <= r^2, on the reference's unfused d^2, drops them.
+// (The neighbour count, which needs the reference's test on every pair, is taken by the force
+// walk: this walk is bound by VALU issue, the force walk by its row gathers.)
+template <bool kCut>
 __device__ __forceinline__ float density_pair(const float4& pi, const float4& q,
-                                              const StepConsts& c, float half_inv_h, float acc,
-                                              uint32_t& cnt) {
-    const float d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
-    const bool in = d2 <= c.r2;  // the reference's test
-    if (kCount) cnt += in ? 1u : 0u;
+                                              const StepConsts& c, float half_inv_h, float acc) {
+    float d2;
+    if (kCut) {
+        d2 = dist2_exact(pi.x, pi.y, pi.z, q.x, q.y, q.z);
+    } else {
+        const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+        d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    }
     float w = w6_shape8(__builtin_amdgcn_sqrtf(d2), half_inv_h, c.inv_h);
-    if (kCut) w = in ? w : 0.0f;
+    if (kCut) w = d2 <= c.r2 ? w : 0.0f;
     return __builtin_fmaf(q.w, w, acc);
 }
 
 // Neighbour positions are requested two steps ahead, in three rotating register sets.
-template <bool kCount, bool kCut, int kSeg>
+template <bool kCut, int kSeg>
 __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                               const uint32_t* __restrict__ lmask,
                                               const uint32_t* __restrict__ lwj0, uint32_t lane,
                                               const float4* __restrict__ spm, const float4& pi,
-                                              const StepConsts& c, float acc, uint32_t& cnt) {
+                                              const StepConsts& c, float acc) {
     if (steps == 0) return acc;
     const float hh = 0.5f * c.inv_h;
     float4 q0 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
@@ -132,14 +136,14 @@ __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
     uint32_t t = 0;
     for (; t + 3 <= steps; t += 3) {
         const float4 q2 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
-        acc = density_pair<kCount, kCut>(pi, q0, c, hh, acc, cnt);
+        acc = density_pair<kCut>(pi, q0, c, hh, acc);
         q0 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
-        acc = density_pair<kCount, kCut>(pi, q1, c, hh, acc, cnt);
+        acc = density_pair<kCut>(pi, q1, c, hh, acc);
         q1 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
-        acc = density_pair<kCount, kCut>(pi, q2, c, hh, acc, cnt);
+        acc = density_pair<kCut>(pi, q2, c, hh, acc);
     }
-    if (t < steps) acc = density_pair<kCount, kCut>(pi, q0, c, hh, acc, cnt);
-    if (t + 1 < steps) acc = density_pair<kCount, kCut>(pi, q1, c, hh, acc, cnt);
+    if (t < steps) acc = density_pair<kCut>(pi, q0, c, hh, acc);
+    if (t + 1 < steps) acc = density_pair<kCut>(pi, q1, c, hh, acc);
     return acc;
 }
 
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
     ForceRow* __restrict__ rows, const uint32_t* __restrict__ skey,
     const uint32_t* __restrict__ cs, StepConsts c, const SceneGrid* __restrict__ scenes,
     uint32_t* __restrict__ gmask, uint32_t* __restrict__ gwj0, uint32_t* __restrict__ gnw,
-    uint32_t* __restrict__ gpart, PartSums ps, unsigned long long* __restrict__ stats) {
+    uint32_t* __restrict__ gpart, PartSums ps) {
     constexpr int kSeg = Shape::seg;
     __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
     __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
@@ -201,7 +205,6 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
     // per-part running sum `cur`, parts folded into `total` as (p0 + p1) + p2
     float total = 0.0f, cur = 0.0f;
     int cur_part = 0;
-    uint32_t cnt = 0;
     int seg_base = wt.first;
     int ti = 0;  // next table entry
     for (;;) {
@@ -224,10 +227,7 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
         BitWalk<kSeg> bw;
         bw.begin(sl.nonempty, lmask, lwj0, lane);
         const uint32_t steps = wave_max(sl.count);
-        if (stats || kSplit)
-            cur = density_bits<true, kCut>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur, cnt);
-        else
-            cur = density_bits<false, kCut>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur, cnt);
+        cur = density_bits<kCut>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur);
         seg_base += nw;
         wave_sync();
         if (tab ? ti >= wt.n : cu.done) break;
@@ -235,7 +235,6 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
     if (kSplit) {
         if (active) {
             ps.acc[(size_t)part * ps.cap + s] = cur;
-            ps.cnt[(size_t)part * ps.cap + s] = cnt;
         }
         return;
     }
@@ -244,26 +243,17 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
         cur = 0.0f;
     }
     if (active) density_epilogue(total, pi, s, c, svr, rows);
-    if (stats) add_nbr_stats(cnt, stats, lane);
 }
 
 // Finishes split density passes: per finite particle (sorted slots below the non-finite
 // bin), the three part sums in order, then the same epilogue as the single-wavefront pass.
 __global__ __launch_bounds__(256) void k_density_finish(
     const float4* __restrict__ spm, float4* __restrict__ svr, ForceRow* __restrict__ rows,
-    const uint32_t* __restrict__ cs, StepConsts c, PartSums ps,
-    unsigned long long* __restrict__ stats) {
+    const uint32_t* __restrict__ cs, StepConsts c, PartSums ps) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t nfin = cs[c.G];
-    const bool active = s < nfin;
-    uint32_t cnt = 0;
-    if (active) {
-        const float a0 = ps.acc[s], a1 = ps.acc[ps.cap + s], a2 = ps.acc[2 * ps.cap + s];
-        density_epilogue((a0 + a1) + a2, spm[s], s, c, svr, rows);
-        cnt = ps.cnt[s] + ps.cnt[ps.cap + s] + ps.cnt[2 * ps.cap + s];
-    }
-    if (stats) add_nbr_stats(cnt, stats, lane);
+    if (s >= cs[c.G]) return;
+    const float a0 = ps.acc[s], a1 = ps.acc[ps.cap + s], a2 = ps.acc[2 * ps.cap + s];
+    density_epilogue((a0 + a1) + a2, spm[s], s, c, svr, rows);
 }
 
 // ---------------------------------------------------------------------------
@@ -337,16 +327,16 @@ struct ForceSums {
     float3 fp = make_float3(0.f, 0.f, 0.f);
     float3 fv = make_float3(0.f, 0.f, 0.f);
     template <bool kCut>
+    // `cnt` counts the pairs that pass the reference's test d^2 <= r^2 on the reference's
+    // unfused d^2 (the particle itself included): the engine's neighbour statistics.
     __device__ __forceinline__ void add(const float4& pi, const float3& vi, float p_quarter,
-                                        const float4& a, const float4& b, const StepConsts& c) {
+                                        const float4& a, const float4& b, const StepConsts& c,
+                                        uint32_t& cnt) {
         const float rx = pi.x - a.x, ry = pi.y - a.y, rz = pi.z - a.z;
-        float d2;
-        if (kCut) {
-            d2 = dist2_exact(pi.x, pi.y, pi.z, a.x, a.y, a.z);
-            d2 = d2 <= c.r2 ? d2 : 1e30f;
-        } else {
-            d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
-        }
+        float d2 = dist2_exact(pi.x, pi.y, pi.z, a.x, a.y, a.z);
+        const bool in = d2 <= c.r2;
+        cnt += in ? 1u : 0u;
+        if (kCut) d2 = in ? d2 : 1e30f;
         const float rinv = __builtin_amdgcn_rsqf(fmaxf(d2, 1e-30f));
         const float rl = d2 * rinv;
         // u = (2 - q) / 2 and s = 1 - q, clamped to [0, 1] by the instructions' clamp bit
@@ -380,7 +370,7 @@ __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
                                            const uint32_t* __restrict__ lwj0, uint32_t lane,
                                            const ForceRow* __restrict__ rows, const float4& pi,
                                            const float3& vi, float p_quarter, const StepConsts& c,
-                                           ForceSums& f) {
+                                           ForceSums& f, uint32_t& cnt) {
     if (steps == 0) return;
     static_assert(sizeof(ForceRow) == 32, "rows are 32-byte blocks");
     uint32_t o = bw.template next<5>(lmask, lwj0, lane);
@@ -391,18 +381,18 @@ __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
     for (; t + 3 <= steps; t += 3) {
         o = bw.template next<5>(lmask, lwj0, lane);
         const float4 a2 = row_at(rows, o).a, b2 = row_at(rows, o).b;
-        f.template add<kCut>(pi, vi, p_quarter, a0, b0, c);
+        f.template add<kCut>(pi, vi, p_quarter, a0, b0, c, cnt);
         o = bw.template next<5>(lmask, lwj0, lane);
         a0 = row_at(rows, o).a;
         b0 = row_at(rows, o).b;
-        f.template add<kCut>(pi, vi, p_quarter, a1, b1, c);
+        f.template add<kCut>(pi, vi, p_quarter, a1, b1, c, cnt);
         o = bw.template next<5>(lmask, lwj0, lane);
         a1 = row_at(rows, o).a;
         b1 = row_at(rows, o).b;
-        f.template add<kCut>(pi, vi, p_quarter, a2, b2, c);
+        f.template add<kCut>(pi, vi, p_quarter, a2, b2, c, cnt);
     }
-    if (t < steps) f.template add<kCut>(pi, vi, p_quarter, a0, b0, c);
-    if (t + 1 < steps) f.template add<kCut>(pi, vi, p_quarter, a1, b1, c);
+    if (t < steps) f.template add<kCut>(pi, vi, p_quarter, a0, b0, c, cnt);
+    if (t + 1 < steps) f.template add<kCut>(pi, vi, p_quarter, a1, b1, c, cnt);
 }
 
 // Folds the sums of finished parts into the running total, which starts at +0:
@@ -476,7 +466,7 @@ void k_force_walk(
     const float* __restrict__ dt_dev, float dt_arg,
     float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc_out, float* __restrict__ rho_out,
-    float* __restrict__ prs_out) {
+    float* __restrict__ prs_out, unsigned long long* __restrict__ stats) {
     constexpr int kSeg = Shape::seg;
     __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
     __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
@@ -509,6 +499,7 @@ void k_force_walk(
     const float P_i = c.k_gas * (rho_i - c.rho0);
     const float p_quarter = 0.25f * P_i;
     ForceSums f;
+    uint32_t cnt = 0;  // neighbours of this wavefront's part(s)
     int cur_part = 0;
     const int total_words = (int)uni(gnw[item]);
     const bool stored = total_words <= kMaxWords;
@@ -566,7 +557,7 @@ void k_force_walk(
         BitWalk<kSeg> bw;
         bw.begin(sl.nonempty, lmask, lwj0, lane);
         force_bits<kSeg, kCut>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi, p_quarter, c,
-                         f);
+                         f, cnt);
         seg_base += nw;
         wave_sync();
         if (!stored && cu.done) break;
@@ -576,9 +567,11 @@ void k_force_walk(
             float4* out = ps.force + 2 * ((size_t)part * ps.cap + s);
             out[0] = make_float4(f.fp.x, f.fp.y, f.fp.z, 0.0f);
             out[1] = make_float4(f.fv.x, f.fv.y, f.fv.z, 0.0f);
+            ps.cnt[(size_t)part * ps.cap + s] = cnt;
         }
         return;
     }
+    if (stats) add_nbr_stats(cnt, stats, lane);
     if (kRegTot)
         fold_parts_reg(rtot, f, kParts - cur_part);
     else
@@ -603,9 +596,16 @@ __global__ __launch_bounds__(256) void k_force_finish(
     const uint32_t* __restrict__ sid, const uint32_t* __restrict__ cs, StepConsts c,
     const SceneGrid* __restrict__ scenes, PartSums ps, const float* __restrict__ dt_dev,
     float dt_arg, float4* __restrict__ pm, float4* __restrict__ vel,
-    float4* __restrict__ acc_out, float* __restrict__ rho_out, float* __restrict__ prs_out) {
+    float4* __restrict__ acc_out, float* __restrict__ rho_out, float* __restrict__ prs_out,
+    unsigned long long* __restrict__ stats) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= cs[c.G]) return;
+    const bool active = s < cs[c.G];
+    if (stats) {
+        const uint32_t cnt =
+            active ? ps.cnt[s] + ps.cnt[ps.cap + s] + ps.cnt[2 * ps.cap + s] : 0u;
+        add_nbr_stats(cnt, stats, threadIdx.x & 63u);
+    }
+    if (!active) return;
     // ((0 + part 0) + part 1) + part 2, as the single-wavefront walk folds them
     float t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -775,8 +775,8 @@ hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const u
                                float4* svr,
                                ForceRow* rows, const uint32_t* skey,
                                const uint32_t* cell_start, uint32_t n,
-                               const StepConsts& c, const SceneGrid* scenes,
-                               unsigned long long* nbr_stats, int shape, hipStream_t st) {
+                               const StepConsts& c, const SceneGrid* scenes, int shape,
+                               hipStream_t st) {
     if (n == 0) return hipSuccess;
     PartSums ps;
     if (hipError_t err = part_sums(wb, n, shape, &ps)) return err;
@@ -787,7 +787,7 @@ hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const u
                                                   dim3(S::threads), 0, st, wb.items,         \
                                                   wb.nitems, spm, sq, svr, rows, skey,       \
                                                   cell_start, c, scenes, wb.mask, wb.wj0,    \
-                                                  wb.nwords, wb.pstarts, ps, nbr_stats))
+                                                  wb.nwords, wb.pstarts, ps))
     if (c.cut) {
         SPHK_DENSITY_LAUNCH(true);
     } else {
@@ -796,7 +796,7 @@ hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const u
 #undef SPHK_DENSITY_LAUNCH
     if (walk_shape_split(shape))
         hipLaunchKernelGGL(k_density_finish, dim3((n + 255) / 256), dim3(256), 0, st, spm, svr,
-                           rows, cell_start, c, ps, nbr_stats);
+                           rows, cell_start, c, ps);
     return hipGetLastError();
 }
 
@@ -807,7 +807,7 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
                              const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
                              float4* vel, float4* acc, float* rho, float* prs, int shape,
-                             float* time_dev, hipStream_t st) {
+                             float* time_dev, unsigned long long* nbr_stats, hipStream_t st) {
     if (n == 0) return time_dev ? launch_advance_time(time_dev, dt_dev, dt, st) : hipSuccess;
     PartSums ps;
     if (hipError_t err = part_sums(wb, n, shape, &ps)) return err;
@@ -819,7 +819,7 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
                                                   wb.nitems, spm, sq, svr, rows, skey, sid,  \
                                                   cell_start, c, scenes, wb.mask, wb.wj0,    \
                                                   wb.nwords, wb.pstarts, ps, dt_dev, dt, pm, \
-                                                  vel, acc, rho, prs))
+                                                  vel, acc, rho, prs, nbr_stats))
     if (c.cut) {
         SPHK_FORCE_LAUNCH(true);
     } else {
@@ -828,7 +828,8 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
 #undef SPHK_FORCE_LAUNCH
     if (walk_shape_split(shape))
         hipLaunchKernelGGL(k_force_finish, dim3((n + 255) / 256), dim3(256), 0, st, spm, svr,
-                           sid, cell_start, c, scenes, ps, dt_dev, dt, pm, vel, acc, rho, prs);
+                           sid, cell_start, c, scenes, ps, dt_dev, dt, pm, vel, acc, rho, prs,
+                           nbr_stats);
     // the non-finite bin holds at most n particles; the kernel reads its start on the device
     hipLaunchKernelGGL(k_nonfinite_step, dim3((n + 255) / 256), dim3(256), 0, st, spm,
                        const_cast<float4*>(svr), sid, cell_start, n, c, scenes, dt_dev, dt, pm, vel,

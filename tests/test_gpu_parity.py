@@ -275,10 +275,10 @@ def test_stored_candidate_words_hold_every_pair(sphk, oracle, scenes, scene_kind
     assert np.isin(bt * n + bidx.astype(np.int64), codes).all(), "a neighbour was filtered out"
     ids = np.arange(n, dtype=np.int64)
     assert np.isin(ids * n + ids, codes).all(), "a particle is not its own candidate"
-    slack = (len(codes) - n - len(bidx)) / len(bidx)
+    slack = (len(codes) - len(bidx)) / len(bidx)  # brute-force lists hold the particle itself
     print(f"{scene_kind} {shape}: {len(bidx)} neighbour pairs, candidates beyond the radius "
           f"{slack:.3f} of them")
-    assert 0.0 <= slack < 0.05  # measured 0.006 (lattice) and 0.001 (config 0)
+    assert 0.0 <= slack < 0.05  # measured 0.011 (lattice) and 0.006 (config 0)
 
 
 def test_far_from_origin_against_brute_force(sphk, oracle):
