@@ -147,6 +147,41 @@ __device__ __forceinline__ float density_bits(BitWalk<kSeg>& bw, uint32_t steps,
     return acc;
 }
 
+// The same walk with kAhead rows in flight, for the split shapes, whose wavefronts are few
+// (below one per SIMD at 10,000 particles) and hide little of each other's gather latency.
+// Rows rotate through kAhead + 1 register sets, each refilled right after its row is used;
+// the pairs are summed in the same order as above.  Measured at 10,000 particles: each walk
+// ~1 us (2 %) faster than with two rows in flight, so the small-scene walks are not bound by
+// gather latency either.
+template <bool kCut, int kSeg, int kAhead>
+__device__ __forceinline__ float density_bits_deep(BitWalk<kSeg>& bw, uint32_t steps,
+                                                   const uint32_t* __restrict__ lmask,
+                                                   const uint32_t* __restrict__ lwj0,
+                                                   uint32_t lane, const float4* __restrict__ spm,
+                                                   const float4& pi, const StepConsts& c,
+                                                   float acc) {
+    if (steps == 0) return acc;
+    const float hh = 0.5f * c.inv_h;
+    float4 q[kAhead + 1];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) q[k] = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
+    uint32_t t = 0;
+    for (; t + kAhead + 1 <= steps; t += kAhead + 1) {
+#pragma unroll
+        for (int k = 0; k <= kAhead; ++k) {
+            q[(k + kAhead) % (kAhead + 1)] = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
+            acc = density_pair<kCut>(pi, q[k], c, hh, acc);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k)
+        if (t + k < steps) acc = density_pair<kCut>(pi, q[k], c, hh, acc);
+    return acc;
+}
+
+// Rows in flight per wavefront in the split shapes' walks.
+constexpr int kDeepAhead = 5;
+
 // Density, the force rows of the particle (B = m/rho and C = B P / 4, so that
 // m_j (P_i + P_j) / (2 rho_j) = 2 ((P_i / 4) B_j + C_j); the 2 rides on dW/dq) and rho in
 // svr.w.
@@ -167,8 +202,10 @@ __device__ __forceinline__ void density_epilogue(float sum, const float4& pi, ui
 // three wavefronts per chunk, the sums finished by k_density_finish).  Seven wavefronts per
 // SIMD: the 15-word shapes' LDS allows it, and the walk is VALU-bound (a few spills outside
 // the walk loops; measured faster at 1M than the 6 the default register allocation gives).
+// The split shapes serve small scenes, whose wavefronts are few: no occupancy target, so
+// their deeper read-ahead stays in registers.
 template <class Shape, bool kSplit, bool kCut>
-__global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_density_walk(
+__global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(kSplit ? 1 : 7, 8))) void k_density_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const uint2* __restrict__ sq, float4* __restrict__ svr,
     ForceRow* __restrict__ rows, const uint32_t* __restrict__ skey,
@@ -227,7 +264,11 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
         BitWalk<kSeg> bw;
         bw.begin(sl.nonempty, lmask, lwj0, lane);
         const uint32_t steps = wave_max(sl.count);
-        cur = density_bits<kCut>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur);
+        if constexpr (kSplit)
+            cur = density_bits_deep<kCut, kSeg, kDeepAhead>(bw, steps, lmask, lwj0, lane, spm, pi,
+                                                            c, cur);
+        else
+            cur = density_bits<kCut>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur);
         seg_base += nw;
         wave_sync();
         if (tab ? ti >= wt.n : cu.done) break;
@@ -395,6 +436,38 @@ __device__ __forceinline__ void force_bits(BitWalk<kSeg>& bw, uint32_t steps,
     if (t + 1 < steps) f.template add<kCut>(pi, vi, p_quarter, a1, b1, c, cnt);
 }
 
+// force_bits with kAhead rows in flight (the split shapes; see density_bits_deep).
+template <int kSeg, bool kCut, int kAhead>
+__device__ __forceinline__ void force_bits_deep(BitWalk<kSeg>& bw, uint32_t steps,
+                                                const uint32_t* __restrict__ lmask,
+                                                const uint32_t* __restrict__ lwj0, uint32_t lane,
+                                                const ForceRow* __restrict__ rows,
+                                                const float4& pi, const float3& vi,
+                                                float p_quarter, const StepConsts& c,
+                                                ForceSums& f, uint32_t& cnt) {
+    if (steps == 0) return;
+    float4 a[kAhead + 1], b[kAhead + 1];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+        const uint32_t o = bw.template next<5>(lmask, lwj0, lane);
+        a[k] = row_at(rows, o).a;
+        b[k] = row_at(rows, o).b;
+    }
+    uint32_t t = 0;
+    for (; t + kAhead + 1 <= steps; t += kAhead + 1) {
+#pragma unroll
+        for (int k = 0; k <= kAhead; ++k) {
+            const uint32_t o = bw.template next<5>(lmask, lwj0, lane);
+            a[(k + kAhead) % (kAhead + 1)] = row_at(rows, o).a;
+            b[(k + kAhead) % (kAhead + 1)] = row_at(rows, o).b;
+            f.template add<kCut>(pi, vi, p_quarter, a[k], b[k], c, cnt);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k)
+        if (t + k < steps) f.template add<kCut>(pi, vi, p_quarter, a[k], b[k], c, cnt);
+}
+
 // Folds the sums of finished parts into the running total, which starts at +0:
 // ((0 + part 0) + part 1) + part 2.  Closing `n` parts at once adds `f` for the first and
 // +0 for each further (empty) one, as the three-wavefront finish does.  The single-wavefront
@@ -556,8 +629,12 @@ void k_force_walk(
         wave_sync();
         BitWalk<kSeg> bw;
         bw.begin(sl.nonempty, lmask, lwj0, lane);
-        force_bits<kSeg, kCut>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi, p_quarter, c,
-                         f, cnt);
+        if constexpr (kSplit)
+            force_bits_deep<kSeg, kCut, kDeepAhead>(bw, wave_max(sl.count), lmask, lwj0, lane,
+                                                    rows, pi, vi, p_quarter, c, f, cnt);
+        else
+            force_bits<kSeg, kCut>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi,
+                                   p_quarter, c, f, cnt);
         seg_base += nw;
         wave_sync();
         if (!stored && cu.done) break;
