@@ -169,6 +169,28 @@ def time_reset_steps(e, dt, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def pool_line(sphk, sph_scenes, shapes, steps=10):
+    """Drop into pool, 9,614,147 particles (400 x 60 x 400 pool + a lattice sphere falling at
+    2 m/s), reset-and-step with the 1M scene's tuned walk shapes."""
+    scene = sph_scenes.config2()
+    e = sphk.Engine(scene.n, scene.params(sphk))
+    e.set_walk_shapes(*shapes)
+    e.upload(scene.pos, scene.vel)
+    e.snapshot_save()
+    t = time_reset_steps(e, scene.dt, steps, 2)
+    e.reset_stats()
+    e.snapshot_restore()
+    e.step(scene.dt)
+    st = e.stats()
+    out = {"particles": scene.n, "steps": steps, "ms_per_step": t * 1e3, "value": scene.n / t,
+           "unit": "particle-updates/s",
+           "mean_neighbors": st.total_neighbors / max(st.total_neighbor_queries, 1),
+           "max_neighbors": st.max_neighbors,
+           "walk_shapes": [sphk.walk_shape_names()[i] for i in shapes]}
+    del e
+    return out
+
+
 def ensemble_line(sphk, sph_scenes, scale, steps):
     """64 independent dam-break scenes (log-uniform 10k..500k particles) stepped together
     through the segmented cell grid, graph-captured, autotuned."""
@@ -559,6 +581,8 @@ def main():
                         {"density": tune["density_ms"], "force": tune["force_ms"]}},
     }
     if not args.no_extras:
+        if args.config == 1:
+            out["drop_into_pool_9.6M"] = pool_line(sphk, sph_scenes, e.walk_shapes())
         out["ensemble"] = ensemble_line(sphk, sph_scenes, args.ensemble_scale, 5)
         out["small_scene_10k"] = graph_line(sphk, sph_scenes, 50)
     if args.full_run:

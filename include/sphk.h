@@ -212,10 +212,11 @@ int sphk_debug_grid(sphk_engine* e, int32_t* origin3, int32_t* dims3, uint64_t* 
 
 /* Launch shapes of the two neighbour walks (LDS words per segment x wavefronts per
  * workgroup; "_split" shapes walk each chunk with three wavefronts, one per part of its
- * candidate columns, and sum the parts in a second kernel), indexed
- * 0..sphk_walk_shape_count()-1.  Results do not depend on the shape (each particle's sums
- * run in the same order in every shape); only speed does.  Until set, an engine uses
- * seg32_waves4_split below 65,536 particles and seg32_waves4 above (chosen again whenever
+ * candidate columns, and sum the parts in a second kernel; "pp" shapes hold two segments,
+ * so a lane through one goes on into the next), indexed 0..sphk_walk_shape_count()-1.
+ * Results do not depend on the shape (each particle's sums run in the same order in every
+ * shape); only speed does.  Until set, an engine uses seg32_waves4_split below 65,536
+ * particles and seg24_waves4 (density) with pp16_waves4 (force) above (chosen again whenever
  * the particle count changes).  No reference counterpart: chosen by the autotuner of the
  * `sph` package. */
 int sphk_set_walk_shapes(sphk_engine* e, int density_shape, int force_shape);

@@ -382,8 +382,11 @@ int ensure_grid(sphk_engine* e) {
         wb.chunk_cap = want;
         ++e->buffer_gen;
     }
-    if (!e->shapes_explicit)
-        e->shape_density = e->shape_force = e->n < kSplitBelow ? kShapeSmall : kShapeLarge;
+    if (!e->shapes_explicit) {
+        const bool small = e->n < kSplitBelow;
+        e->shape_density = small ? kShapeSmall : kShapeLargeDensity;
+        e->shape_force = small ? kShapeSmall : kShapeLargeForce;
+    }
     // per-part sums of the split walk shapes: three sums per slot, only while selected
     if ((walk_shape_split(e->shape_density) || walk_shape_split(e->shape_force)) &&
         wb.part_cap < e->cap) {

@@ -81,18 +81,20 @@ constexpr uint64_t kMaxCapacity = (1ull << 27) - 2;
 
 // neighbour walks: words of 32 candidates kept in LDS per wave, and per chunk in HBM
 constexpr int kMaxWords = 128;
-constexpr int kNumWalkShapes = 7;  // launch shapes selectable per walk (see walk_shape_name)
+constexpr int kNumWalkShapes = 9;  // launch shapes selectable per walk (see walk_shape_name)
 const char* walk_shape_name(int v);
 // Parts of a chunk's candidate columns (corners, edges, centre with the outer ring).
 constexpr int kParts = 3;
 // Shapes that walk each chunk with three wavefronts, one per part of its candidate columns;
 // they need the per-part sums of WalkBuffers.
-constexpr bool walk_shape_split(int v) { return v >= 5; }
+constexpr bool walk_shape_split(int v) { return v == 5 || v == 6; }
 // Shapes an engine uses until the caller sets its own: below kSplitBelow particles the
 // single-wavefront walk leaves most of the GPU idle and the split seg32_waves4 is faster;
-// above, seg32_waves4 (dam-break blocks of 10K to 1M particles on MI355X).
+// above, seg24_waves4 for the density walk and pp16_waves4 for the force walk (the fastest
+// pair on the 1M dam break on MI355X).
 constexpr int kShapeSmall = 6;
-constexpr int kShapeLarge = 1;
+constexpr int kShapeLargeDensity = 4;
+constexpr int kShapeLargeForce = 8;
 constexpr uint64_t kSplitBelow = 65536;
 
 // HBM buffers of the chunked walks (sized for max_chunks())

@@ -251,14 +251,9 @@ k_sur_forward(WalkBuffers wb, const uint32_t* __restrict__ batch, uint32_t nbatc
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t col = lane & 31u, h = lane >> 5;
     WalkLds& L = lds_all[half];
-    const uint32_t b_idx = swizzled_block();
-    uint32_t item;
-    if (batch) {
-        if (b_idx >= nbatch) return;
-        item = uni(batch[b_idx]);
-    } else {
-        item = b_idx;
-    }
+    const uint32_t b_idx = xcd_unit(batch ? nbatch : uni(wb.nitems[0]), 1, 0);
+    if (b_idx == ~0u) return;
+    const uint32_t item = batch ? uni(batch[b_idx]) : b_idx;
     const uint32_t clane = 32u * half + col;  // this lane's target within the chunk
     Chunk ch;
     uint32_t s;
@@ -830,7 +825,7 @@ hipError_t launch_sur_forward(bool integrate, const WalkBuffers& wb, const uint3
                               hipStream_t st) {
     const uint32_t chunks = batch ? nbatch : max_items;
     if (chunks == 0) return hipSuccess;
-    const uint32_t blocks = (chunks + kXcds - 1) / kXcds * kXcds;  // swizzled_block() needs it
+    const uint32_t blocks = (chunks + kXcds - 1) / kXcds * kXcds;  // xcd_unit() needs it
     if (integrate) {
         hipLaunchKernelGGL(k_sur_forward<true>, dim3(blocks), dim3(64 * kEdgeWaves), 0, st, wb,
                            batch, nbatch, spm, svr, skey, sid, cs, c, scenes, pk, pf, sc, rows,

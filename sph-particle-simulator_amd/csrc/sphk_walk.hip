@@ -182,6 +182,34 @@ __device__ __forceinline__ float density_bits_deep(BitWalk<kSeg>& bw, uint32_t s
 // Rows in flight per wavefront in the split shapes' walks.
 constexpr int kDeepAhead = 5;
 
+// The ping-pong shapes' walk, which goes on across calls: on entry and on return the rows of
+// the lane's next two pairs are in flight in q0 and q1, and a call takes exactly `steps` bits
+// and sums exactly `steps` pairs (the two in flight at a part's end are summed by the caller).
+template <bool kCut, int kW>
+__device__ __forceinline__ float density_run(BitWalk<kW>& bw, float4& q0, float4& q1,
+                                             uint32_t steps, const uint32_t* __restrict__ lmask,
+                                             const uint32_t* __restrict__ lwj0, uint32_t lane,
+                                             const float4* __restrict__ spm, const float4& pi,
+                                             const StepConsts& c, float acc) {
+    const float hh = 0.5f * c.inv_h;
+    uint32_t t = 0;
+    for (; t + 3 <= steps; t += 3) {
+        const float4 q2 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
+        acc = density_pair<kCut>(pi, q0, c, hh, acc);
+        q0 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
+        acc = density_pair<kCut>(pi, q1, c, hh, acc);
+        q1 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
+        acc = density_pair<kCut>(pi, q2, c, hh, acc);
+    }
+    for (; t < steps; ++t) {
+        const float4 q2 = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
+        acc = density_pair<kCut>(pi, q0, c, hh, acc);
+        q0 = q1;
+        q1 = q2;
+    }
+    return acc;
+}
+
 // Density, the force rows of the particle (B = m/rho and C = B P / 4, so that
 // m_j (P_i + P_j) / (2 rho_j) = 2 ((P_i / 4) B_j + C_j); the 2 rides on dW/dq) and rho in
 // svr.w.
@@ -203,9 +231,10 @@ __device__ __forceinline__ void density_epilogue(float sum, const float4& pi, ui
 // SIMD: the 15-word shapes' LDS allows it, and the walk is VALU-bound (a few spills outside
 // the walk loops; measured faster at 1M than the 6 the default register allocation gives).
 // The split shapes serve small scenes, whose wavefronts are few: no occupancy target, so
-// their deeper read-ahead stays in registers.
+// their deeper read-ahead stays in registers.  The ping-pong shapes aim at what their LDS
+// allows (6 with two 12-word buffers, 4 with two 16-word ones).
 template <class Shape, bool kSplit, bool kCut>
-__global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(kSplit ? 1 : 7, 8))) void k_density_walk(
+__global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(kSplit ? 1 : Shape::density_waves, 8))) void k_density_walk(
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ nitems,
     const float4* __restrict__ spm, const uint2* __restrict__ sq, float4* __restrict__ svr,
     ForceRow* __restrict__ rows, const uint32_t* __restrict__ skey,
@@ -213,13 +242,17 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
     uint32_t* __restrict__ gmask, uint32_t* __restrict__ gwj0, uint32_t* __restrict__ gnw,
     uint32_t* __restrict__ gpart, PartSums ps) {
     constexpr int kSeg = Shape::seg;
-    __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
-    __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
+    constexpr int kLds = Shape::lds_words;
+    static_assert(!(kSplit && Shape::pp), "ping-pong shapes walk a chunk with one wavefront");
+    __shared__ uint32_t lmask_all[Shape::waves][kLds * 64];
+    __shared__ uint32_t lwj0_all[Shape::waves][kLds];
     const uint32_t wave = uni(threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t* lmask = lmask_all[wave];
     uint32_t* lwj0 = lwj0_all[wave];
-    const uint32_t wid = swizzled_block() * Shape::waves + wave;
+    const uint32_t units = kSplit ? uni(nitems[0]) * kParts : uni(nitems[0]);
+    const uint32_t wid = xcd_unit(units, Shape::waves, wave);
+    if (wid == ~0u) return;
     const uint32_t item = kSplit ? wid / kParts : wid;
     const int part = kSplit ? (int)(wid % kParts) : -1;
     Chunk ch;
@@ -228,7 +261,7 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
     float4 pi;
     if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
     const uint2 qi = active ? sq[s] : make_uint2(0u, 0u);
-    init_sentinel<kSeg>(lmask, lwj0, lane, c);
+    init_sentinel<kLds>(lmask, lwj0, lane, c);
     MfmaFilter mf;
     mfma_filter_begin(mf, ch, sq, qi, lane, c.qneg);
     WordTable wt;
@@ -244,34 +277,111 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
     int cur_part = 0;
     int seg_base = wt.first;
     int ti = 0;  // next table entry
-    for (;;) {
-        SegLane sl;
-        int seg_part;
-        const int nw =
-            tab ? phase1_table<kSeg>(wt, ti, sq, c, qi, active, lmask, lwj0, lane, sl, seg_part,
-                                     mf)
-                : phase1_segment<kSeg, true>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane,
-                                             sl, seg_part, &mf);
-        if (nw == 0) break;
-        if (!kSplit) {
-            for (; cur_part < seg_part; ++cur_part) {
+    if constexpr (Shape::pp) {
+        // Segments are made in the newer buffer.  A segment of the part being walked joins
+        // the walk at once; one of the next part waits there until every lane is through the
+        // current part (the parts' sums are folded between them, in the same order as in
+        // every other shape).
+        uint32_t* const lm_new = lmask + kSeg * 64;
+        uint32_t* const lj_new = lwj0 + kSeg;
+        const float hh = 0.5f * c.inv_h;
+        PingPong<kSeg> pp;
+        pp.init(c.pad_slot);
+        float4 q0, q1;  // rows of the next two pairs, requested
+        bool more = true, pend = false;
+        SegLane pend_sl = {0u, 0u};
+        int pend_part = 0, pend_nw = 0, walk_part = -1;
+        for (;;) {
+            if (pp.nw_new == 0 && !pend && more) {
+                SegLane sl;
+                int seg_part;
+                const int nw = tab ? phase1_table<kSeg>(wt, ti, sq, c, qi, active, lm_new, lj_new,
+                                                        lane, sl, seg_part, mf)
+                                   : phase1_segment<kSeg, true>(cu, ch, sq, cs, c, qi, active,
+                                                                lm_new, lj_new, lane, sl,
+                                                                seg_part, &mf);
+                if (nw == 0) {
+                    more = false;
+                } else {
+                    wave_sync();
+                    store_words(item, seg_base, nw, lm_new, lj_new, gmask, gwj0, lane);
+                    seg_base += nw;
+                    if (tab ? ti >= wt.n : cu.done) more = false;
+                    if (walk_part < 0 || seg_part == walk_part) {
+                        pp.add_newer(sl, nw, lmask, lwj0, lane, c.pad_slot);
+                        if (walk_part < 0) {  // the walk's first segment
+                            for (; cur_part < seg_part; ++cur_part) {
+                                total = cur_part == 0 ? cur : total + cur;
+                                cur = 0.0f;
+                            }
+                            walk_part = seg_part;
+                            q0 = row_at(spm, pp.bw.template next<4>(lmask, lwj0, lane));
+                            q1 = row_at(spm, pp.bw.template next<4>(lmask, lwj0, lane));
+                            pp.walked(2);
+                        }
+                    } else {
+                        pend = true;
+                        pend_sl = sl;
+                        pend_part = seg_part;
+                        pend_nw = nw;
+                    }
+                }
+            }
+            if (pp.nw_new != 0) {
+                const uint32_t steps = pp.older_steps();
+                cur = density_run<kCut>(pp.bw, q0, q1, steps, lmask, lwj0, lane, spm, pi, c, cur);
+                pp.walked(steps);
+                pp.shift_down(lmask, lwj0, lane);
+                continue;
+            }
+            if (walk_part < 0) break;  // no segment at all
+            const uint32_t steps = pp.all_steps();
+            cur = density_run<kCut>(pp.bw, q0, q1, steps, lmask, lwj0, lane, spm, pi, c, cur);
+            pp.walked(steps);
+            cur = density_pair<kCut>(pi, q0, c, hh, cur);  // the part's last two pairs
+            cur = density_pair<kCut>(pi, q1, c, hh, cur);
+            if (!pend) break;
+            for (; cur_part < pend_part; ++cur_part) {
                 total = cur_part == 0 ? cur : total + cur;
                 cur = 0.0f;
             }
+            walk_part = pend_part;
+            pend = false;
+            pp.add_newer(pend_sl, pend_nw, lmask, lwj0, lane, c.pad_slot);
+            q0 = row_at(spm, pp.bw.template next<4>(lmask, lwj0, lane));
+            q1 = row_at(spm, pp.bw.template next<4>(lmask, lwj0, lane));
+            pp.walked(2);
         }
-        wave_sync();
-        store_words(item, seg_base, nw, lmask, lwj0, gmask, gwj0, lane);
-        BitWalk<kSeg> bw;
-        bw.begin(sl.nonempty, lmask, lwj0, lane);
-        const uint32_t steps = wave_max(sl.count);
-        if constexpr (kSplit)
-            cur = density_bits_deep<kCut, kSeg, kDeepAhead>(bw, steps, lmask, lwj0, lane, spm, pi,
-                                                            c, cur);
-        else
-            cur = density_bits<kCut>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur);
-        seg_base += nw;
-        wave_sync();
-        if (tab ? ti >= wt.n : cu.done) break;
+    } else {
+        for (;;) {
+            SegLane sl;
+            int seg_part;
+            const int nw =
+                tab ? phase1_table<kSeg>(wt, ti, sq, c, qi, active, lmask, lwj0, lane, sl, seg_part,
+                                         mf)
+                    : phase1_segment<kSeg, true>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane,
+                                                 sl, seg_part, &mf);
+            if (nw == 0) break;
+            if (!kSplit) {
+                for (; cur_part < seg_part; ++cur_part) {
+                    total = cur_part == 0 ? cur : total + cur;
+                    cur = 0.0f;
+                }
+            }
+            wave_sync();
+            store_words(item, seg_base, nw, lmask, lwj0, gmask, gwj0, lane);
+            BitWalk<kSeg> bw;
+            bw.begin(sl.nonempty, lmask, lwj0, lane);
+            const uint32_t steps = wave_max(sl.count);
+            if constexpr (kSplit)
+                cur = density_bits_deep<kCut, kSeg, kDeepAhead>(bw, steps, lmask, lwj0, lane, spm,
+                                                                pi, c, cur);
+            else
+                cur = density_bits<kCut>(bw, steps, lmask, lwj0, lane, spm, pi, c, cur);
+            seg_base += nw;
+            wave_sync();
+            if (tab ? ti >= wt.n : cu.done) break;
+        }
     }
     if (kSplit) {
         if (active) {
@@ -468,6 +578,42 @@ __device__ __forceinline__ void force_bits_deep(BitWalk<kSeg>& bw, uint32_t step
         if (t + k < steps) f.template add<kCut>(pi, vi, p_quarter, a[k], b[k], c, cnt);
 }
 
+// force_bits for the ping-pong shapes: the walk goes on across calls with the rows of the next
+// two pairs in flight (a0/b0, a1/b1) on entry and on return; a call takes exactly `steps`
+// bits and adds exactly `steps` pairs.
+template <int kW, bool kCut>
+__device__ __forceinline__ void force_run(BitWalk<kW>& bw, float4& a0, float4& b0, float4& a1,
+                                          float4& b1, uint32_t steps,
+                                          const uint32_t* __restrict__ lmask,
+                                          const uint32_t* __restrict__ lwj0, uint32_t lane,
+                                          const ForceRow* __restrict__ rows, const float4& pi,
+                                          const float3& vi, float p_quarter, const StepConsts& c,
+                                          ForceSums& f, uint32_t& cnt) {
+    uint32_t t = 0;
+    for (; t + 3 <= steps; t += 3) {
+        uint32_t o = bw.template next<5>(lmask, lwj0, lane);
+        const float4 a2 = row_at(rows, o).a, b2 = row_at(rows, o).b;
+        f.template add<kCut>(pi, vi, p_quarter, a0, b0, c, cnt);
+        o = bw.template next<5>(lmask, lwj0, lane);
+        a0 = row_at(rows, o).a;
+        b0 = row_at(rows, o).b;
+        f.template add<kCut>(pi, vi, p_quarter, a1, b1, c, cnt);
+        o = bw.template next<5>(lmask, lwj0, lane);
+        a1 = row_at(rows, o).a;
+        b1 = row_at(rows, o).b;
+        f.template add<kCut>(pi, vi, p_quarter, a2, b2, c, cnt);
+    }
+    for (; t < steps; ++t) {
+        const uint32_t o = bw.template next<5>(lmask, lwj0, lane);
+        const float4 a2 = row_at(rows, o).a, b2 = row_at(rows, o).b;
+        f.template add<kCut>(pi, vi, p_quarter, a0, b0, c, cnt);
+        a0 = a1;
+        b0 = b1;
+        a1 = a2;
+        b1 = b2;
+    }
+}
+
 // Folds the sums of finished parts into the running total, which starts at +0:
 // ((0 + part 0) + part 1) + part 2.  Closing `n` parts at once adds `f` for the first and
 // +0 for each further (empty) one, as the three-wavefront finish does.  The single-wavefront
@@ -541,18 +687,22 @@ void k_force_walk(
     float4* __restrict__ vel, float4* __restrict__ acc_out, float* __restrict__ rho_out,
     float* __restrict__ prs_out, unsigned long long* __restrict__ stats) {
     constexpr int kSeg = Shape::seg;
-    __shared__ uint32_t lmask_all[Shape::waves][kSeg * 64];
-    __shared__ uint32_t lwj0_all[Shape::waves][kSeg];
+    constexpr int kLds = Shape::lds_words;
+    static_assert(!(kSplit && Shape::pp), "ping-pong shapes walk a chunk with one wavefront");
+    __shared__ uint32_t lmask_all[Shape::waves][kLds * 64];
+    __shared__ uint32_t lwj0_all[Shape::waves][kLds];
     const uint32_t wave = uni(threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
-    constexpr bool kRegTot = !kSplit && kSeg >= 24;
+    constexpr bool kRegTot = !kSplit && !Shape::pp && kSeg >= 24;
     constexpr bool kLdsTot = !kSplit && !kRegTot;
     __shared__ float ltot_all[kLdsTot ? Shape::waves : 1][kLdsTot ? 6 * 64 : 1];
     float rtot[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     uint32_t* lmask = lmask_all[wave];
     uint32_t* lwj0 = lwj0_all[wave];
     float* ltot = ltot_all[kLdsTot ? wave : 0];
-    const uint32_t wid = swizzled_block() * Shape::waves + wave;
+    const uint32_t units = kSplit ? uni(nitems[0]) * kParts : uni(nitems[0]);
+    const uint32_t wid = xcd_unit(units, Shape::waves, wave);
+    if (wid == ~0u) return;
     const uint32_t item = kSplit ? wid / kParts : wid;
     const int part = kSplit ? (int)(wid % kParts) : -1;
     Chunk ch;
@@ -561,7 +711,7 @@ void k_force_walk(
     float4 pi;
     if (!load_chunk(item, items, nitems, skey, cs, spm, c, scenes, lane, ch, s, active, pi)) return;
     const uint2 qi = active ? sq[s] : make_uint2(0u, 0u);
-    init_sentinel<kSeg>(lmask, lwj0, lane, c);
+    init_sentinel<kLds>(lmask, lwj0, lane, c);
     if (kLdsTot) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) ltot[k * 64 + lane] = 0.0f;
@@ -585,59 +735,157 @@ void k_force_walk(
         seg_base = part_word_start(pstarts, part);
         words_end = part_word_end(pstarts, part, total_words);
     }
-    for (;;) {
-        int nw, seg_part;
-        SegLane sl;
-        if (stored) {
-            if (seg_base >= words_end) break;
-            seg_part = part_of_word(seg_base, pstarts);
-            const int pend = part_word_end(pstarts, seg_part, total_words);
-            nw = min(kSeg - 1, pend - seg_base);
-            const size_t base = (size_t)item * kMaxWords + seg_base;
-            sl.nonempty = 0;
-            sl.count = 0;
-            // all of the segment's word loads are issued before the first is used (a loop
-            // over nw would wait for each load of its remainder in turn); slots past nw
-            // re-read word nw - 1 and are dropped.  The words are read once: nontemporal
-            // loads keep them from evicting the neighbour rows from L2
-            uint32_t wv[kSeg - 1];
+    if constexpr (Shape::pp) {
+        // As in the density walk: segments are loaded into the newer buffer; one of the part
+        // being walked joins the walk at once, one of the next part waits until every lane
+        // is through the current part.
+        uint32_t* const lm_new = lmask + kSeg * 64;
+        uint32_t* const lj_new = lwj0 + kSeg;
+        PingPong<kSeg> pp;
+        pp.init(c.pad_slot);
+        float4 a0, b0, a1, b1;  // rows of the next two pairs, requested
+        bool more = true, pend = false;
+        SegLane pend_sl = {0u, 0u};
+        int pend_part = 0, pend_nw = 0, walk_part = -1;
+        auto request_two = [&]() {
+            uint32_t o = pp.bw.template next<5>(lmask, lwj0, lane);
+            a0 = row_at(rows, o).a;
+            b0 = row_at(rows, o).b;
+            o = pp.bw.template next<5>(lmask, lwj0, lane);
+            a1 = row_at(rows, o).a;
+            b1 = row_at(rows, o).b;
+            pp.walked(2);
+        };
+        for (;;) {
+            if (pp.nw_new == 0 && !pend && more) {
+                int nw = 0, seg_part = 0;
+                SegLane sl = {0u, 0u};
+                if (stored) {
+                    if (seg_base < words_end) {
+                        seg_part = part_of_word(seg_base, pstarts);
+                        const int pend_end = part_word_end(pstarts, seg_part, total_words);
+                        nw = min(kSeg - 1, pend_end - seg_base);
+                        const size_t base = (size_t)item * kMaxWords + seg_base;
+                        uint32_t wv[kSeg - 1];
 #pragma unroll
-            for (int w = 0; w < kSeg - 1; ++w)
-                wv[w] = __builtin_nontemporal_load(
-                    &gmask[(base + (size_t)min(w, nw - 1)) * 64 + lane]);
+                        for (int w = 0; w < kSeg - 1; ++w)
+                            wv[w] = __builtin_nontemporal_load(
+                                &gmask[(base + (size_t)min(w, nw - 1)) * 64 + lane]);
 #pragma unroll
-            for (int w = 0; w < kSeg - 1; ++w) {
-                if (w < nw) {
-                    lmask[w * 64 + lane] = wv[w];
-                    sl.nonempty |= (wv[w] != 0u ? 1u : 0u) << w;
-                    sl.count += (uint32_t)__builtin_popcount(wv[w]);
+                        for (int w = 0; w < kSeg - 1; ++w) {
+                            if (w < nw) {
+                                lm_new[w * 64 + lane] = wv[w];
+                                sl.nonempty |= (wv[w] != 0u ? 1u : 0u) << w;
+                                sl.count += (uint32_t)__builtin_popcount(wv[w]);
+                            }
+                        }
+                        for (int w = (int)lane; w < nw; w += 64) lj_new[w] = gwj0[base + w];
+                        seg_base += nw;
+                    }
+                    if (seg_base >= words_end) more = false;
+                } else {
+                    nw = phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lm_new, lj_new, lane,
+                                              sl, seg_part);
+                    if (cu.done) more = false;
+                }
+                if (nw == 0) {
+                    more = false;
+                } else {
+                    wave_sync();
+                    if (walk_part < 0 || seg_part == walk_part) {
+                        pp.add_newer(sl, nw, lmask, lwj0, lane, c.pad_slot);
+                        if (walk_part < 0) {  // the walk's first segment
+                            fold_parts(ltot, lane, f, seg_part - cur_part);
+                            cur_part = seg_part;
+                            walk_part = seg_part;
+                            request_two();
+                        }
+                    } else {
+                        pend = true;
+                        pend_sl = sl;
+                        pend_part = seg_part;
+                        pend_nw = nw;
+                    }
                 }
             }
-            for (int w = (int)lane; w < nw; w += 64) lwj0[w] = gwj0[base + w];
-        } else {
-            nw = phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane, sl,
-                                      seg_part);
-            if (nw == 0) break;
+            if (pp.nw_new != 0) {
+                const uint32_t steps = pp.older_steps();
+                force_run<kLds, kCut>(pp.bw, a0, b0, a1, b1, steps, lmask, lwj0, lane, rows, pi,
+                                      vi, p_quarter, c, f, cnt);
+                pp.walked(steps);
+                pp.shift_down(lmask, lwj0, lane);
+                continue;
+            }
+            if (walk_part < 0) break;  // no segment at all
+            const uint32_t steps = pp.all_steps();
+            force_run<kLds, kCut>(pp.bw, a0, b0, a1, b1, steps, lmask, lwj0, lane, rows, pi, vi,
+                                  p_quarter, c, f, cnt);
+            pp.walked(steps);
+            f.template add<kCut>(pi, vi, p_quarter, a0, b0, c, cnt);  // the part's last two pairs
+            f.template add<kCut>(pi, vi, p_quarter, a1, b1, c, cnt);
+            if (!pend) break;
+            fold_parts(ltot, lane, f, pend_part - cur_part);
+            cur_part = pend_part;
+            walk_part = pend_part;
+            pend = false;
+            pp.add_newer(pend_sl, pend_nw, lmask, lwj0, lane, c.pad_slot);
+            request_two();
         }
-        if (!kSplit) {
-            if (kRegTot)
-                fold_parts_reg(rtot, f, seg_part - cur_part);
+    } else {
+        for (;;) {
+            int nw, seg_part;
+            SegLane sl;
+            if (stored) {
+                if (seg_base >= words_end) break;
+                seg_part = part_of_word(seg_base, pstarts);
+                const int pend = part_word_end(pstarts, seg_part, total_words);
+                nw = min(kSeg - 1, pend - seg_base);
+                const size_t base = (size_t)item * kMaxWords + seg_base;
+                sl.nonempty = 0;
+                sl.count = 0;
+                // all of the segment's word loads are issued before the first is used (a loop
+                // over nw would wait for each load of its remainder in turn); slots past nw
+                // re-read word nw - 1 and are dropped.  The words are read once: nontemporal
+                // loads keep them from evicting the neighbour rows from L2
+                uint32_t wv[kSeg - 1];
+    #pragma unroll
+                for (int w = 0; w < kSeg - 1; ++w)
+                    wv[w] = __builtin_nontemporal_load(
+                        &gmask[(base + (size_t)min(w, nw - 1)) * 64 + lane]);
+    #pragma unroll
+                for (int w = 0; w < kSeg - 1; ++w) {
+                    if (w < nw) {
+                        lmask[w * 64 + lane] = wv[w];
+                        sl.nonempty |= (wv[w] != 0u ? 1u : 0u) << w;
+                        sl.count += (uint32_t)__builtin_popcount(wv[w]);
+                    }
+                }
+                for (int w = (int)lane; w < nw; w += 64) lwj0[w] = gwj0[base + w];
+            } else {
+                nw = phase1_segment<kSeg>(cu, ch, sq, cs, c, qi, active, lmask, lwj0, lane, sl,
+                                          seg_part);
+                if (nw == 0) break;
+            }
+            if (!kSplit) {
+                if (kRegTot)
+                    fold_parts_reg(rtot, f, seg_part - cur_part);
+                else
+                    fold_parts(ltot, lane, f, seg_part - cur_part);
+                cur_part = seg_part;
+            }
+            wave_sync();
+            BitWalk<kSeg> bw;
+            bw.begin(sl.nonempty, lmask, lwj0, lane);
+            if constexpr (kSplit)
+                force_bits_deep<kSeg, kCut, kDeepAhead>(bw, wave_max(sl.count), lmask, lwj0, lane,
+                                                        rows, pi, vi, p_quarter, c, f, cnt);
             else
-                fold_parts(ltot, lane, f, seg_part - cur_part);
-            cur_part = seg_part;
+                force_bits<kSeg, kCut>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi,
+                                       p_quarter, c, f, cnt);
+            seg_base += nw;
+            wave_sync();
+            if (!stored && cu.done) break;
         }
-        wave_sync();
-        BitWalk<kSeg> bw;
-        bw.begin(sl.nonempty, lmask, lwj0, lane);
-        if constexpr (kSplit)
-            force_bits_deep<kSeg, kCut, kDeepAhead>(bw, wave_max(sl.count), lmask, lwj0, lane,
-                                                    rows, pi, vi, p_quarter, c, f, cnt);
-        else
-            force_bits<kSeg, kCut>(bw, wave_max(sl.count), lmask, lwj0, lane, rows, pi, vi,
-                                   p_quarter, c, f, cnt);
-        seg_base += nw;
-        wave_sync();
-        if (!stored && cu.done) break;
     }
     if (kSplit) {
         if (active) {
@@ -765,7 +1013,9 @@ __global__ __launch_bounds__(Shape::threads) void k_nbr_lists(
     uint32_t s;
     bool active;
     float4 pi;
-    if (!load_chunk(swizzled_block() * Shape::waves + wave, items, nitems, skey, cs, spm, c, scenes,
+    const uint32_t wid = xcd_unit(uni(nitems[0]), Shape::waves, wave);
+    if (wid == ~0u) return;
+    if (!load_chunk(wid, items, nitems, skey, cs, spm, c, scenes,
                     lane, ch, s, active, pi))
         return;
     const uint2 qi = active ? sq[s] : make_uint2(0u, 0u);
@@ -804,6 +1054,8 @@ using Shape1 = WalkShape<32, 4>;
 using Shape2 = WalkShape<28, 4>;
 using Shape3 = WalkShape<32, 1>;
 using Shape4 = WalkShape<24, 4>;
+using Shape5 = WalkShape<12, 4, true>;
+using Shape6 = WalkShape<16, 4, true>;
 
 const char* walk_shape_name(int v) {
     switch (v) {
@@ -814,13 +1066,16 @@ const char* walk_shape_name(int v) {
         case 4: return "seg24_waves4";
         case 5: return "seg16_waves4_split";
         case 6: return "seg32_waves4_split";
+        case 7: return "pp12_waves4";
+        case 8: return "pp16_waves4";
         default: return "?";
     }
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The grid is
-// padded to a multiple of 8 and block b works on block (b % 8) * (grid / 8) + b / 8, so each
-// XCD walks one contiguous run of chunks, whose candidate columns overlap.  A split shape
+// padded to a multiple of 8 and sized for the most chunks a step can have (the host does not
+// know the count, which the chunk list leaves on the device); each XCD walks one contiguous
+// eighth of the chunks that exist (xcd_unit), whose candidate columns overlap.  A split shape
 // runs three wavefronts per chunk (consecutive, so mostly in one workgroup).
 template <class Shape>
 static uint32_t grid_for(uint32_t chunks, bool split = false) {
@@ -838,6 +1093,8 @@ static uint32_t grid_for(uint32_t chunks, bool split = false) {
         case 3: { using S = Shape3; constexpr bool kS = false; CALL; break; } \
         case 5: { using S = Shape0; constexpr bool kS = true; CALL; break; } \
         case 6: { using S = Shape1; constexpr bool kS = true; CALL; break; } \
+        case 7: { using S = Shape5; constexpr bool kS = false; CALL; break; } \
+        case 8: { using S = Shape6; constexpr bool kS = false; CALL; break; } \
         default: { using S = Shape4; constexpr bool kS = false; CALL; break; } \
     }
 
@@ -922,8 +1179,9 @@ hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint
                             int shape, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const uint32_t chunks = max_chunks(n, c);
-    // one wavefront per chunk in every shape (a split shape lists with its unsplit twin)
-    const int base = shape == 5 ? 0 : shape == 6 ? 1 : shape;
+    // one wavefront per chunk in every shape (a split shape lists with its unsplit twin, a
+    // ping-pong shape with the single-buffer shape of the same LDS size)
+    const int base = shape == 5 ? 0 : shape == 6 ? 1 : shape == 7 ? 4 : shape == 8 ? 1 : shape;
     SPHK_DISPATCH_SHAPE(base, hipLaunchKernelGGL(k_nbr_lists<S>, dim3(grid_for<S>(chunks)),
                                                   dim3(S::threads), 0, st, wb.items, wb.nitems,
                                                   spm, sq, skey, sid, cell_start, c, scenes,

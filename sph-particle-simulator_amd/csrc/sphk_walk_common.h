@@ -125,10 +125,18 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_re
 
 constexpr uint32_t kXcds = 8;
 
-// Block index after the XCD swizzle (see grid_for).
-__device__ __forceinline__ uint32_t swizzled_block() {
-    const uint32_t per = gridDim.x / kXcds;
-    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+// Work unit of this wavefront (a chunk, or a chunk's part in the split walks) after the XCD
+// split: workgroups are dealt to the 8 XCDs in turn (block b runs on XCD b % 8), and XCD x
+// takes the x-th eighth of the `units` units that exist (one contiguous run, so neighbouring
+// chunks share its L2).  The grid is sized for the most units a step can have (see grid_for),
+// which gives every XCD at least an eighth of them; ~0u for a wavefront past its XCD's run.
+__device__ __forceinline__ uint32_t xcd_unit(uint32_t units, uint32_t waves_per_block,
+                                             uint32_t wave) {
+    const uint32_t run = (units + kXcds - 1) / kXcds;
+    const uint32_t j = (blockIdx.x / kXcds) * waves_per_block + wave;
+    if (j >= run) return ~0u;
+    const uint32_t u = (blockIdx.x % kXcds) * run + j;
+    return u < units ? u : ~0u;
 }
 __device__ __forceinline__ int unii(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -805,6 +813,64 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t x) {
     return uni(x);
 }
 
+// Ping-pong segments of a wavefront: the older segment's words at LDS words [0, kSeg - 1),
+// the newer one's at [kSeg, 2 kSeg - 1), the sentinel at word 2 kSeg - 1; the bit walk runs
+// over both (its set holds older words in bits 0.., newer ones in bits kSeg..), so a lane
+// through the older segment goes straight on into the newer.  `rem` counts a lane's bits
+// not yet walked: every step takes one (the read-ahead word is always the next real one, see
+// add_newer), so the wavefront knows how many steps empty the older buffer.
+template <int kSeg>
+struct PingPong {
+    static constexpr uint32_t kSentinel = 1u << (2 * kSeg - 1);
+    static constexpr uint32_t kLow = (1u << (kSeg - 1)) - 1u;
+    BitWalk<2 * kSeg> bw;
+    uint32_t rem = 0;      // this lane's bits not yet walked, both buffers
+    uint32_t cnt_new = 0;  // this lane's bits in the newer buffer
+    int nw_new = 0;        // uniform: words in the newer buffer (0: free)
+    __device__ __forceinline__ void init(uint32_t pad_slot) {
+        bw.m = 0u;
+        bw.jb = 0u;
+        bw.ne = kSentinel;
+        bw.mq = 0x80000000u;  // read-ahead: the sentinel word
+        bw.jq = pad_slot;
+    }
+    // A segment of `nw` words written to the newer buffer joins the walk; a lane whose
+    // read-ahead is the sentinel reads again, so no step is lost on it.
+    __device__ __forceinline__ void add_newer(const SegLane& sl, int nw,
+                                              const uint32_t* __restrict__ lmask,
+                                              const uint32_t* __restrict__ lwj0, uint32_t lane,
+                                              uint32_t pad_slot) {
+        bw.ne |= sl.nonempty << kSeg;
+        rem += sl.count;
+        cnt_new = sl.count;
+        nw_new = nw;
+        if (bw.jq == pad_slot) bw.fetch(lmask, lwj0, lane);
+    }
+    // Steps after which every lane is through the older buffer; steps that walk all.
+    __device__ __forceinline__ uint32_t older_steps() const {
+        return wave_max(rem > cnt_new ? rem - cnt_new : 0u);
+    }
+    __device__ __forceinline__ uint32_t all_steps() const { return wave_max(rem); }
+    __device__ __forceinline__ void walked(uint32_t steps) { rem = rem > steps ? rem - steps : 0u; }
+    // Every lane is through the older buffer: the newer segment's words move down to it.
+    // All kSeg - 1 words are copied (those past the segment are never read: no bit of `ne`
+    // names them), every read issued before the first write.
+    __device__ __forceinline__ void shift_down(uint32_t* __restrict__ lmask,
+                                               uint32_t* __restrict__ lwj0, uint32_t lane) {
+        wave_sync();
+        uint32_t v[kSeg - 1];
+#pragma unroll
+        for (int w = 0; w < kSeg - 1; ++w) v[w] = lmask[(kSeg + w) * 64 + lane];
+        const uint32_t j = lwj0[kSeg + (lane < kSeg - 1 ? lane : 0u)];
+#pragma unroll
+        for (int w = 0; w < kSeg - 1; ++w) lmask[w * 64 + lane] = v[w];
+        if (lane < kSeg - 1) lwj0[lane] = j;
+        bw.ne = ((bw.ne >> kSeg) & kLow) | kSentinel;
+        nw_new = 0;
+        wave_sync();
+    }
+};
+
 constexpr int kCols = 32;       // target columns of a wavefront (two lanes per target)
 constexpr int kListCap = 72;    // non-empty words per target column held at once
 constexpr int kBatch = 16;      // stored words requested per batch while building a list
@@ -960,14 +1026,22 @@ __device__ __forceinline__ void walk_pairs(uint32_t item, const Chunk& ch, uint3
 }
 
 
-template <int kSeg, int kWaves>
+// kPP: two buffers of kSeg words each (ping-pong): while the wavefront walks the older
+// segment, the next one is already in the newer buffer, so a lane that is through the older
+// segment goes on with the newer instead of waiting for the slowest lane.
+template <int kSeg, int kWaves, bool kPP = false>
 struct WalkShape {
     static constexpr int seg = kSeg;
     static constexpr int waves = kWaves;
     static constexpr int threads = 64 * kWaves;
+    static constexpr bool pp = kPP;
+    static constexpr int lds_words = kPP ? 2 * kSeg : kSeg;  // LDS words per wavefront
+    static_assert(!kPP || 2 * kSeg <= 32, "both buffers' words fit one 32-bit set");
     // occupancy target of the force walk: 5 wavefronts per SIMD where the LDS allows it
     // (24- and 28-word segments without the LDS running total), else what the registers give
-    static constexpr int force_waves = (kSeg == 24 || kSeg == 28) ? 5 : 1;
+    static constexpr int force_waves = (!kPP && (kSeg == 24 || kSeg == 28)) ? 5 : 1;
+    // occupancy target of the single-wavefront density walk
+    static constexpr int density_waves = !kPP ? 7 : kSeg <= 12 ? 6 : 4;
 };
 
 }  // namespace

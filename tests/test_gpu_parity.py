@@ -245,7 +245,8 @@ def _stored_pairs(e, n):
 
 
 @pytest.mark.parametrize("scene_kind", ["lattice", "config0"])
-@pytest.mark.parametrize("shape", ["seg32_waves4", "seg24_waves4", "seg32_waves4_split"])
+@pytest.mark.parametrize("shape", ["seg32_waves4", "seg24_waves4", "seg32_waves4_split",
+                                   "pp12_waves4"])
 def test_stored_candidate_words_hold_every_pair(sphk, oracle, scenes, scene_kind, shape):
     # The words the force walk reads are the matrix-core filter's output in the production
     # density walk.  Pair by pair: every reference neighbour (brute force, d^2 <= r^2 with the
@@ -483,11 +484,14 @@ def test_walk_shapes_bitexact(sphk, scenes):
     # mix freely with the single-wavefront ones, density and force chosen independently)
     # Both scenes leave candidate columns empty at the block's faces (an empty column must
     # not hide the start of the next column group from the segment and part breaks).
+    # (the ping-pong shapes walk two segment buffers, lanes running ahead into the newer one)
     names = sphk.walk_shape_names()
     split = [v for v, nm in enumerate(names) if nm.endswith("_split")]
-    assert split, names
+    pp = [v for v, nm in enumerate(names) if nm.startswith("pp")]
+    assert split and pp, names
     pairs = [(v, v) for v in range(len(names))]
-    pairs += [(split[0], 0), (0, split[0]), (split[-1], 2)]
+    pairs += [(split[0], 0), (0, split[0]), (split[-1], 2), (pp[0], pp[-1]), (pp[-1], split[0]),
+              (split[-1], pp[0])]
     for s, dt in ((scenes.config0(seed=21), None), (scenes.small_block(10, 10, 10, seed=8), 1e-4)):
         dt = dt or s.dt
         ref = None
@@ -511,7 +515,7 @@ def test_walk_shapes_bitexact(sphk, scenes):
                 np.testing.assert_array_equal(idx, ref_n[1])
 
 
-@pytest.mark.parametrize("split", [False, True])
+@pytest.mark.parametrize("split", [False, True, "pp"])
 def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes, split):
     # a pile-up cell with more than 4096 candidates per chunk exceeds the stored-word budget;
     # the force pass then re-tests.  Both paths must agree with the oracle, in the
@@ -521,8 +525,10 @@ def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes, split):
                           rng.uniform(0.3, 0.5, size=(500, 3))]).astype(np.float32)
     p = sphk.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
     e = sphk.Engine(len(pos), p)
-    if split:
-        names = sphk.walk_shape_names()
+    names = sphk.walk_shape_names()
+    if split == "pp":
+        e.set_walk_shapes(names.index("pp12_waves4"), names.index("pp16_waves4"))
+    elif split:
         v = names.index("seg16_waves4_split")
         e.set_walk_shapes(v, v)
     e.upload(pos)
@@ -553,7 +559,8 @@ def test_pileup_split_and_single_walks_bitexact(sphk):
     names = sphk.walk_shape_names()
     outs = []
     for d, f in ((0, 0), (names.index("seg16_waves4_split"), names.index("seg32_waves4_split")),
-                 (1, 0)):
+                 (1, 0), (names.index("pp12_waves4"), names.index("pp16_waves4")),
+                 (names.index("pp16_waves4"), names.index("pp12_waves4"))):
         p = sphk.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
         e = sphk.Engine(len(pos), p)
         e.set_walk_shapes(d, f)
