@@ -287,9 +287,8 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
         const float hh = 0.5f * c.inv_h;
         PingPong<kSeg> pp;
         pp.init(c.pad_slot);
-        float4 q0, q1;  // rows of the next two pairs, requested
-        bool more = true, pend = false;
-        SegLane pend_sl = {0u, 0u};
+        float4 q0, q1;  // rows of the next two pairs, requested (within a phase)
+        bool more = true, pend = false;  // pend: the newer buffer holds the next part's segment
         int pend_part = 0, pend_nw = 0, walk_part = -1;
         for (;;) {
             if (pp.nw_new == 0 && !pend && more) {
@@ -308,38 +307,41 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
                     seg_base += nw;
                     if (tab ? ti >= wt.n : cu.done) more = false;
                     if (walk_part < 0 || seg_part == walk_part) {
-                        pp.add_newer(sl, nw, lmask, lwj0, lane, c.pad_slot);
                         if (walk_part < 0) {  // the walk's first segment
                             for (; cur_part < seg_part; ++cur_part) {
                                 total = cur_part == 0 ? cur : total + cur;
                                 cur = 0.0f;
                             }
                             walk_part = seg_part;
-                            q0 = row_at(spm, pp.bw.template next<4>(lmask, lwj0, lane));
-                            q1 = row_at(spm, pp.bw.template next<4>(lmask, lwj0, lane));
-                            pp.walked(2);
                         }
+                        pp.add_newer(sl, nw, lmask, lwj0, lane, c.pad_slot);
                     } else {
                         pend = true;
-                        pend_sl = sl;
                         pend_part = seg_part;
                         pend_nw = nw;
                     }
                 }
             }
-            if (pp.nw_new != 0) {
-                const uint32_t steps = pp.older_steps();
+            // A phase: walk until every lane is through the older buffer (or, with nothing to
+            // overlap, through all it holds).  Its read-ahead starts at the phase's start and its
+            // last two rows are summed at its end, in walk order; nothing stays in flight while
+            // the next segment is made.
+            const bool overlap = pp.nw_new != 0;
+            if (!overlap && walk_part < 0) break;  // no segment at all
+            if ((overlap ? pp.older_steps() : pp.all_steps()) != 0) {
+                q0 = row_at(spm, pp.bw.template next<4>(lmask, lwj0, lane));
+                q1 = row_at(spm, pp.bw.template next<4>(lmask, lwj0, lane));
+                pp.walked(2);
+                const uint32_t steps = overlap ? pp.older_steps() : pp.all_steps();
                 cur = density_run<kCut>(pp.bw, q0, q1, steps, lmask, lwj0, lane, spm, pi, c, cur);
                 pp.walked(steps);
+                cur = density_pair<kCut>(pi, q0, c, hh, cur);
+                cur = density_pair<kCut>(pi, q1, c, hh, cur);
+            }
+            if (overlap) {
                 pp.shift_down(lmask, lwj0, lane);
                 continue;
             }
-            if (walk_part < 0) break;  // no segment at all
-            const uint32_t steps = pp.all_steps();
-            cur = density_run<kCut>(pp.bw, q0, q1, steps, lmask, lwj0, lane, spm, pi, c, cur);
-            pp.walked(steps);
-            cur = density_pair<kCut>(pi, q0, c, hh, cur);  // the part's last two pairs
-            cur = density_pair<kCut>(pi, q1, c, hh, cur);
             if (!pend) break;
             for (; cur_part < pend_part; ++cur_part) {
                 total = cur_part == 0 ? cur : total + cur;
@@ -347,10 +349,7 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
             }
             walk_part = pend_part;
             pend = false;
-            pp.add_newer(pend_sl, pend_nw, lmask, lwj0, lane, c.pad_slot);
-            q0 = row_at(spm, pp.bw.template next<4>(lmask, lwj0, lane));
-            q1 = row_at(spm, pp.bw.template next<4>(lmask, lwj0, lane));
-            pp.walked(2);
+            pp.add_held(pend_nw, lmask, lwj0, lane, c.pad_slot);
         }
     } else {
         for (;;) {
@@ -743,9 +742,8 @@ void k_force_walk(
         uint32_t* const lj_new = lwj0 + kSeg;
         PingPong<kSeg> pp;
         pp.init(c.pad_slot);
-        float4 a0, b0, a1, b1;  // rows of the next two pairs, requested
-        bool more = true, pend = false;
-        SegLane pend_sl = {0u, 0u};
+        float4 a0, b0, a1, b1;  // rows of the next two pairs, requested (within a phase)
+        bool more = true, pend = false;  // pend: the newer buffer holds the next part's segment
         int pend_part = 0, pend_nw = 0, walk_part = -1;
         auto request_two = [&]() {
             uint32_t o = pp.bw.template next<5>(lmask, lwj0, lane);
@@ -793,43 +791,41 @@ void k_force_walk(
                 } else {
                     wave_sync();
                     if (walk_part < 0 || seg_part == walk_part) {
-                        pp.add_newer(sl, nw, lmask, lwj0, lane, c.pad_slot);
                         if (walk_part < 0) {  // the walk's first segment
                             fold_parts(ltot, lane, f, seg_part - cur_part);
                             cur_part = seg_part;
                             walk_part = seg_part;
-                            request_two();
                         }
+                        pp.add_newer(sl, nw, lmask, lwj0, lane, c.pad_slot);
                     } else {
                         pend = true;
-                        pend_sl = sl;
                         pend_part = seg_part;
                         pend_nw = nw;
                     }
                 }
             }
-            if (pp.nw_new != 0) {
-                const uint32_t steps = pp.older_steps();
+            // A phase, as in the density walk: nothing stays in flight between phases.
+            const bool overlap = pp.nw_new != 0;
+            if (!overlap && walk_part < 0) break;  // no segment at all
+            if ((overlap ? pp.older_steps() : pp.all_steps()) != 0) {
+                request_two();
+                const uint32_t steps = overlap ? pp.older_steps() : pp.all_steps();
                 force_run<kLds, kCut>(pp.bw, a0, b0, a1, b1, steps, lmask, lwj0, lane, rows, pi,
                                       vi, p_quarter, c, f, cnt);
                 pp.walked(steps);
+                f.template add<kCut>(pi, vi, p_quarter, a0, b0, c, cnt);
+                f.template add<kCut>(pi, vi, p_quarter, a1, b1, c, cnt);
+            }
+            if (overlap) {
                 pp.shift_down(lmask, lwj0, lane);
                 continue;
             }
-            if (walk_part < 0) break;  // no segment at all
-            const uint32_t steps = pp.all_steps();
-            force_run<kLds, kCut>(pp.bw, a0, b0, a1, b1, steps, lmask, lwj0, lane, rows, pi, vi,
-                                  p_quarter, c, f, cnt);
-            pp.walked(steps);
-            f.template add<kCut>(pi, vi, p_quarter, a0, b0, c, cnt);  // the part's last two pairs
-            f.template add<kCut>(pi, vi, p_quarter, a1, b1, c, cnt);
             if (!pend) break;
             fold_parts(ltot, lane, f, pend_part - cur_part);
             cur_part = pend_part;
             walk_part = pend_part;
             pend = false;
-            pp.add_newer(pend_sl, pend_nw, lmask, lwj0, lane, c.pad_slot);
-            request_two();
+            pp.add_held(pend_nw, lmask, lwj0, lane, c.pad_slot);
         }
     } else {
         for (;;) {

@@ -846,6 +846,20 @@ struct PingPong {
         nw_new = nw;
         if (bw.jq == pad_slot) bw.fetch(lmask, lwj0, lane);
     }
+    // add_newer for a segment already in the newer buffer whose per-lane bits were not kept
+    // (a segment of the next part, held until the current part is walked).
+    __device__ __forceinline__ void add_held(int nw, const uint32_t* __restrict__ lmask,
+                                             const uint32_t* __restrict__ lwj0, uint32_t lane,
+                                             uint32_t pad_slot) {
+        SegLane sl = {0u, 0u};
+#pragma unroll
+        for (int w = 0; w < kSeg - 1; ++w) {
+            const uint32_t word = w < nw ? lmask[(kSeg + w) * 64 + lane] : 0u;
+            sl.nonempty |= (word != 0u ? 1u : 0u) << w;
+            sl.count += (uint32_t)__builtin_popcount(word);
+        }
+        add_newer(sl, nw, lmask, lwj0, lane, pad_slot);
+    }
     // Steps after which every lane is through the older buffer; steps that walk all.
     __device__ __forceinline__ uint32_t older_steps() const {
         return wave_max(rem > cnt_new ? rem - cnt_new : 0u);
@@ -854,16 +868,21 @@ struct PingPong {
     __device__ __forceinline__ void walked(uint32_t steps) { rem = rem > steps ? rem - steps : 0u; }
     // Every lane is through the older buffer: the newer segment's words move down to it.
     // All kSeg - 1 words are copied (those past the segment are never read: no bit of `ne`
-    // names them), every read issued before the first write.
+    // names them), four at a time, each group's reads issued before its writes.
     __device__ __forceinline__ void shift_down(uint32_t* __restrict__ lmask,
                                                uint32_t* __restrict__ lwj0, uint32_t lane) {
         wave_sync();
-        uint32_t v[kSeg - 1];
-#pragma unroll
-        for (int w = 0; w < kSeg - 1; ++w) v[w] = lmask[(kSeg + w) * 64 + lane];
         const uint32_t j = lwj0[kSeg + (lane < kSeg - 1 ? lane : 0u)];
 #pragma unroll
-        for (int w = 0; w < kSeg - 1; ++w) lmask[w * 64 + lane] = v[w];
+        for (int w0 = 0; w0 < kSeg - 1; w0 += 4) {
+            uint32_t v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (w0 + k < kSeg - 1) v[k] = lmask[(kSeg + w0 + k) * 64 + lane];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (w0 + k < kSeg - 1) lmask[(w0 + k) * 64 + lane] = v[k];
+        }
         if (lane < kSeg - 1) lwj0[lane] = j;
         bw.ne = ((bw.ne >> kSeg) & kLow) | kSentinel;
         nw_new = 0;
