@@ -426,21 +426,31 @@ int total_key_bits(uint64_t G) {
     return b;
 }
 
-// keys -> sort -> cell_start -> gather.  Leaves e->skey/e->sid/e->spm/e->svr valid.
-int build_sorted(sphk_engine* e) {
+// keys -> sort -> cell_start -> gather -> chunk list.  Leaves e->skey/e->sid/e->spm/e->svr
+// valid.  With `tail` (a step's build), the chunk-list launch also steps the particles of the
+// non-finite bin and advances the clock (the fields filled here are taken from the engine).
+int build_sorted(sphk_engine* e, const StepTail* tail) {
     const uint32_t n = (uint32_t)e->n;
     hipStream_t st = e->stream;
-    HIP_TRY(launch_cell_keys(e->pm, n, e->consts, e->d_scenes, e->keys, e->uscal, st));
     uint32_t* ko = nullptr;
     uint32_t* vo = nullptr;
-    HIP_TRY(radix_sort_keys(e->keys, e->keys_alt, e->vals, e->vals_alt, e->hist, e->scan_scratch,
-                            n, total_key_bits(e->consts.G), &ko, &vo, st));
+    HIP_TRY(radix_sort_keys(e->pm, e->consts, e->d_scenes, e->uscal, e->keys, e->keys_alt, e->vals,
+                            e->vals_alt, e->hist, e->scan_scratch, n,
+                            total_key_bits(e->consts.G), &ko, &vo, st));
     e->skey = ko;
     e->sid = vo;
     HIP_TRY(launch_cell_start(e->skey, n, e->consts.G, e->cell_start, st));
     HIP_TRY(launch_gather(e->pm, e->vel, e->sid, n, e->consts.qscale, e->sq, e->spm, e->svr, st));
+    StepTail t;
+    if (tail) {
+        t = *tail;
+        t.spm = e->spm;
+        t.svr = e->svr;
+        t.sid = e->sid;
+        t.n = n;
+    }
     HIP_TRY(launch_chunks(e->cell_start, e->consts, e->d_scenes, e->walk.colcnt, e->scan_scratch,
-                          e->walk.items, e->walk.nitems, st));
+                          e->walk.items, e->walk.nitems, tail ? &t : nullptr, st));
     e->sorted_valid = true;
     return SPHK_OK;
 }
@@ -540,7 +550,16 @@ int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev) {
     const uint32_t n = (uint32_t)e->n;
     if (ev) HIP_TRY(hipEventRecord(ev[0], st));
     if (adaptive) HIP_TRY(launch_cfl(e->vel, e->acc, n, e->params, e->uscal + 1, e->fscal, st));
-    if (int rc = build_sorted(e)) return rc;
+    StepTail tail{};
+    tail.dt_dev = adaptive ? e->fscal : nullptr;
+    tail.dt = dt;
+    tail.pm = e->pm;
+    tail.vel = e->vel;
+    tail.acc = e->acc;
+    tail.rho = e->rho;
+    tail.prs = e->prs;
+    tail.time_dev = e->fscal + 1;
+    if (int rc = build_sorted(e, &tail)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
     HIP_TRY(launch_density_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->cell_start,
                                 n, e->consts, e->d_scenes, e->shape_density, st));
@@ -548,8 +567,7 @@ int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev) {
     HIP_TRY(launch_force_walk(e->walk, e->spm, e->sq, e->svr, e->frows, e->skey, e->sid,
                               e->cell_start, n,
                               e->consts, e->d_scenes, adaptive ? e->fscal : nullptr, dt, e->pm,
-                              e->vel, e->acc, e->rho, e->prs, e->shape_force, e->fscal + 1,
-                              e->nstats, st));
+                              e->vel, e->acc, e->rho, e->prs, e->shape_force, e->nstats, st));
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
     return SPHK_OK;
 }

@@ -151,7 +151,7 @@ int set_device(const sphk_engine* e);
 // Validates the scene table and (re)builds the cell grids when the step needs it.
 int prepare_grid(sphk_engine* e);
 // keys -> sort -> cell_start -> gather -> chunk list for the current state.
-int build_sorted(sphk_engine* e);
+int build_sorted(sphk_engine* e, const StepTail* tail = nullptr);
 // One plain step enqueued on the engine stream (ev: optional 4 timing events).
 int enqueue_step(sphk_engine* e, float dt, bool adaptive, hipEvent_t* ev);
 // Host bookkeeping after a step that moved particles (scene extents, grid validity).

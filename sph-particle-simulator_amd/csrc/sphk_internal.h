@@ -154,15 +154,56 @@ __device__ __forceinline__ uint32_t scene_of_particle(const SceneGrid* __restric
     return lo;
 }
 
+// Exclusive prefix of v over the 256 threads of a workgroup (all threads call it); *total, if
+// given, receives the workgroup's sum.  lds_waves: 4 words of LDS.
+__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* lds_waves,
+                                                             uint32_t* total) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63) lds_waves[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+    for (uint32_t w = 0; w < 4u; ++w) {
+        uint32_t t = lds_waves[w];
+        if (w < wave) wbase += t;
+        tot += t;
+    }
+    if (total) *total = tot;
+    return wbase + x - v;
+}
+
+// Device-side arguments of the step's last part for the particles of the non-finite bin
+// (they have no neighbour: rho = m W(0), a = (0, g, 0), integrated and clamped like the rest),
+// done by the chunk-list launch of a step's build; it also advances the clock by dt.
+struct StepTail {
+    const float4* spm;
+    const float4* svr;
+    const uint32_t* sid;
+    uint32_t n;
+    const float* dt_dev;  // adaptive dt on the device, or null for `dt`
+    float dt;
+    float4* pm;
+    float4* vel;
+    float4* acc;
+    float* rho;
+    float* prs;
+    float* time_dev;
+};
+
 // launchers; all enqueue on `st`
 size_t scan_scratch_words(size_t m);
 hipError_t scan_exclusive(uint32_t* data, uint32_t m, uint32_t* scratch, hipStream_t st);
-hipError_t launch_cell_keys(const float4* pm, uint32_t n, const StepConsts& c,
-                            const SceneGrid* scenes, uint32_t* keys, uint32_t* err_flag,
-                            hipStream_t st);
-hipError_t radix_sort_keys(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
-                           uint32_t* hist, uint32_t* scan_scratch, uint32_t n, int total_bits,
-                           uint32_t** keys_out, uint32_t** vals_out, hipStream_t st);
+// stable LSD sort of the particles' dense cell indices (computed from pm in the first pass and
+// left in `keys`) with their caller ids as values
+hipError_t radix_sort_keys(const float4* pm, const StepConsts& c, const SceneGrid* scenes,
+                           uint32_t* err_flag, uint32_t* keys, uint32_t* keys_alt, uint32_t* vals,
+                           uint32_t* vals_alt, uint32_t* hist, uint32_t* scan_scratch, uint32_t n,
+                           int total_bits, uint32_t** keys_out, uint32_t** vals_out,
+                           hipStream_t st);
 size_t radix_hist_words(uint32_t n, int total_bits);
 hipError_t launch_cell_start(const uint32_t* skey, uint32_t n, uint32_t G, uint32_t* cell_start,
                              hipStream_t st);
@@ -170,9 +211,10 @@ hipError_t launch_gather(const float4* pm, const float4* vel, const uint32_t* si
                          double qscale, uint2* sq,
                          float4* spm, float4* svr, hipStream_t st);
 uint32_t max_chunks(uint32_t n, const StepConsts& c);
+// chunk list of the sorted particles; with `tail`, also the non-finite particles' step
 hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, const SceneGrid* scenes,
                          uint32_t* cnt_scratch, uint32_t* scan_scratch, uint32_t* items,
-                         uint32_t* nitems, hipStream_t st);
+                         uint32_t* nitems, const StepTail* tail, hipStream_t st);
 hipError_t launch_density_walk(const WalkBuffers& wb, const float4* spm, const uint2* sq,
                                float4* svr,
                                ForceRow* rows, const uint32_t* skey,
@@ -186,7 +228,7 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
                              const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
                              float4* vel, float4* acc, float* rho, float* prs, int shape,
-                             float* time_dev, unsigned long long* nbr_stats, hipStream_t st);
+                             unsigned long long* nbr_stats, hipStream_t st);
 hipError_t launch_nbr_lists(const WalkBuffers& wb, const float4* spm, const uint2* sq,
                             const uint32_t* skey,
                             const uint32_t* sid, const uint32_t* cell_start, uint32_t n,

@@ -1,8 +1,8 @@
 // sphk_kernels.hip -- gfx950 kernels of the SPH step.
 //
 // Step pipeline (one HIP stream):
-//   cell_keys      dense cell index of every particle, caller order
-//   radix sort     stable LSD sort of (cell, caller id) -> cell-sorted order
+//   radix sort     stable LSD sort of (cell, caller id) -> cell-sorted order; the first pass
+//                  computes the dense cell index of every particle (caller order)
 //   cell_start     occupancy prefix of the dense grid (binary search per cell)
 //   gather         cell-sorted float4 copies of (x,y,z,m) and (vx,vy,vz,-), quantized
 //                  coordinates of the candidate filter
@@ -21,16 +21,13 @@ namespace sphk {
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // ---------------------------------------------------------------------------
-// K1: dense cell index per particle (caller order)
+// dense cell index per particle (caller order)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_cell_keys(const float4* __restrict__ pm, uint32_t n,
-                                                   StepConsts c,
-                                                   const SceneGrid* __restrict__ scenes,
-                                                   uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ err) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float4 p = pm[i];
+// Dense index of particle i at p: its cell in its scene's grid, or G (the non-finite bin) for a
+// non-finite coordinate.  A finite particle outside the grid's interior sets *err.
+__device__ __forceinline__ uint32_t cell_key(const float4& p, uint32_t i, const StepConsts& c,
+                                             const SceneGrid* __restrict__ scenes,
+                                             uint32_t* __restrict__ err) {
     uint32_t L = c.G;
     if (__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z)) {
         const SceneGrid g = scenes[scene_of_particle(scenes, c.nscenes, i)];
@@ -49,41 +46,61 @@ __global__ __launch_bounds__(256) void k_cell_keys(const float4* __restrict__ pm
             atomicOr(err, 1u);
         }
     }
-    keys[i] = L;
-}
-
-hipError_t launch_cell_keys(const float4* pm, uint32_t n, const StepConsts& c,
-                            const SceneGrid* scenes, uint32_t* keys, uint32_t* err_flag,
-                            hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_keys, dim3((n + 255) / 256), dim3(256), 0, st, pm, n, c, scenes,
-                       keys, err_flag);
-    return hipGetLastError();
+    return L;
 }
 
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort of 32-bit keys with 32-bit values.  Each pass:
 //   upsweep   per-tile digit histogram (LDS atomics)   -> hist[digit][tile]
 //   scan      exclusive scan of hist (digit-major)     -> global base per (digit, tile)
+//             (done inside the downsweep when hist is small)
 //   downsweep per-wave stable ranks from ballot-matched peer groups, per-digit
 //             prefix over the waves of the tile, scatter.
 // Items of a tile are owned wave-contiguously (wave w owns 512 consecutive keys,
 // 64 per iteration), so (wave, iteration, lane) order is input order and every
 // pass is stable.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kRadixThreads) void k_radix_upsweep(const uint32_t* __restrict__ keys,
-                                                                 uint32_t n, int shift,
-                                                                 uint32_t mask, uint32_t ntiles,
-                                                                 uint32_t* __restrict__ hist) {
+// kKeys: the first pass also computes the keys from the positions (cell_key) and stores them
+// for its downsweep, so the keys need no launch of their own.
+template <bool kKeys>
+__global__ __launch_bounds__(kRadixThreads) void k_radix_upsweep(
+    const uint32_t* __restrict__ keys, uint32_t n, int shift, uint32_t mask, uint32_t ntiles,
+    uint32_t* __restrict__ hist, const float4* __restrict__ pm, StepConsts c,
+    const SceneGrid* __restrict__ scenes, uint32_t* __restrict__ keys_out,
+    uint32_t* __restrict__ err) {
     __shared__ uint32_t h[1 << kRadixMaxBits];
     const uint32_t nb = mask + 1;
     for (uint32_t i = threadIdx.x; i < nb; i += kRadixThreads) h[i] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kRadixTile;
+    // all of a thread's loads are issued before its first histogram update
+    uint32_t key[kRadixItems];
+    if constexpr (kKeys) {
+        float4 p[kRadixItems];
+#pragma unroll
+        for (int k = 0; k < kRadixItems; ++k) {
+            const uint32_t idx = base + k * kRadixThreads + threadIdx.x;
+            p[k] = idx < n ? pm[idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < kRadixItems; ++k) {
+            const uint32_t idx = base + k * kRadixThreads + threadIdx.x;
+            if (idx < n) {
+                key[k] = cell_key(p[k], idx, c, scenes, err);
+                keys_out[idx] = key[k];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kRadixItems; ++k) {
+            const uint32_t idx = base + k * kRadixThreads + threadIdx.x;
+            key[k] = idx < n ? keys[idx] : 0u;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < kRadixItems; ++k) {
-        uint32_t idx = base + k * kRadixThreads + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
+        const uint32_t idx = base + k * kRadixThreads + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(key[k] >> shift) & mask], 1u);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nb; i += kRadixThreads) hist[i * ntiles + blockIdx.x] = h[i];
@@ -94,26 +111,6 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_upsweep(const uint32_t*
 constexpr int kScanThreads = 256;
 constexpr int kScanPer = 16;
 constexpr uint32_t kScanTile = kScanThreads * kScanPer;
-
-__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* lds_waves,
-                                                             uint32_t* total) {
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(x, off);
-        if (lane >= (uint32_t)off) x += y;
-    }
-    if (lane == 63) lds_waves[wave] = x;
-    __syncthreads();
-    uint32_t wbase = 0, tot = 0;
-    for (uint32_t w = 0; w < (uint32_t)(kScanThreads / 64); ++w) {
-        uint32_t t = lds_waves[w];
-        if (w < wave) wbase += t;
-        tot += t;
-    }
-    if (total) *total = tot;
-    return wbase + x - v;
-}
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_tile_sums(const uint32_t* __restrict__ data,
                                                                   uint32_t m,
@@ -177,13 +174,44 @@ hipError_t scan_exclusive(uint32_t* data, uint32_t m, uint32_t* scratch, hipStre
     return hipGetLastError();
 }
 
+// Histograms of at most this many words (digits x tiles) are scanned by every downsweep
+// workgroup for itself (kLocalScan), which saves the scan's launches on small inputs.
+constexpr uint32_t kRadixLocalScanMax = 8192;
+
+// kLocalScan: `offs` is the unscanned histogram hist[digit][tile] (at most kRadixLocalScanMax
+// words), and each workgroup takes the exclusive prefix at its own tile's entries itself.
+template <bool kLocalScan>
 __global__ __launch_bounds__(kRadixThreads) void k_radix_downsweep(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
     uint32_t mask, int bits, uint32_t ntiles, const uint32_t* __restrict__ offs) {
     __shared__ uint32_t wc[kRadixWaves << kRadixMaxBits];
+    __shared__ uint32_t loffs[kLocalScan ? (1 << kRadixMaxBits) : 1];
     const uint32_t nb = mask + 1;
     for (uint32_t i = threadIdx.x; i < kRadixWaves * nb; i += kRadixThreads) wc[i] = 0;
+    if constexpr (kLocalScan) {
+        // thread t sums the words [t*per, (t+1)*per) of the digit-major histogram; the
+        // exclusive prefix of the block gives each word's global prefix, kept for the words of
+        // this tile (positions d * ntiles + blockIdx.x)
+        __shared__ uint32_t lsw[kRadixWaves];
+        const uint32_t m = nb * ntiles;
+        const uint32_t per = (m + kRadixThreads - 1) / kRadixThreads;
+        const uint32_t beg = threadIdx.x * per;
+        const uint32_t end = beg + per < m ? beg + per : m;
+        uint32_t sum = 0;
+        for (uint32_t i = beg; i < end; ++i) sum += offs[i];
+        uint32_t run = block_exclusive_scan_256(sum, lsw, nullptr);
+        // first position of this tile at or after beg
+        uint32_t next = beg <= blockIdx.x ? blockIdx.x
+                                          : blockIdx.x + ((beg - blockIdx.x + ntiles - 1) / ntiles) * ntiles;
+        for (uint32_t i = beg; i < end; ++i) {
+            if (i == next) {
+                loffs[i / ntiles] = run;
+                next += ntiles;
+            }
+            run += offs[i];
+        }
+    }
     __syncthreads();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63;
@@ -225,7 +253,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_downsweep(
         uint32_t idx = base + k * 64 + lane;
         if (idx < n) {
             uint32_t d = (key[k] >> shift) & mask;
-            uint32_t dest = offs[d * ntiles + blockIdx.x] + mywc[d] + rank[k];
+            uint32_t dest = (kLocalScan ? loffs[d] : offs[d * ntiles + blockIdx.x]) + mywc[d] + rank[k];
             keys_out[dest] = key[k];
             vals_out[dest] = vals_in ? vals_in[idx] : idx;
         }
@@ -246,9 +274,13 @@ size_t radix_hist_words(uint32_t n, int total_bits) {
     return ((size_t)1 << bits_per) * ntiles;
 }
 
-hipError_t radix_sort_keys(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
-                           uint32_t* hist, uint32_t* scan_scratch, uint32_t n, int total_bits,
-                           uint32_t** keys_out, uint32_t** vals_out, hipStream_t st) {
+// The first pass computes the keys (cell_key of each position) in its upsweep and writes them
+// to `keys`.
+hipError_t radix_sort_keys(const float4* pm, const StepConsts& c, const SceneGrid* scenes,
+                           uint32_t* err_flag, uint32_t* keys, uint32_t* keys_alt, uint32_t* vals,
+                           uint32_t* vals_alt, uint32_t* hist, uint32_t* scan_scratch, uint32_t n,
+                           int total_bits, uint32_t** keys_out, uint32_t** vals_out,
+                           hipStream_t st) {
     int passes, bits_per;
     radix_plan(total_bits, passes, bits_per);
     if (total_bits < 1) total_bits = 1;
@@ -262,12 +294,23 @@ hipError_t radix_sort_keys(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, u
         int b = total_bits - shift < bits_per ? total_bits - shift : bits_per;
         uint32_t mask = (1u << b) - 1u;
         uint32_t m = (mask + 1) * ntiles;
-        hipLaunchKernelGGL(k_radix_upsweep, dim3(ntiles), dim3(kRadixThreads), 0, st, src_k, n,
-                           shift, mask, ntiles, hist);
-        hipError_t se = scan_exclusive(hist, m, scan_scratch, st);
-        if (se != hipSuccess) return se;
-        hipLaunchKernelGGL(k_radix_downsweep, dim3(ntiles), dim3(kRadixThreads), 0, st, src_k,
-                           src_v, dst_k, dst_v, n, shift, mask, b, ntiles, hist);
+        if (p == 0)
+            hipLaunchKernelGGL(k_radix_upsweep<true>, dim3(ntiles), dim3(kRadixThreads), 0, st,
+                               (const uint32_t*)nullptr, n, shift, mask, ntiles, hist, pm, c,
+                               scenes, src_k, err_flag);
+        else
+            hipLaunchKernelGGL(k_radix_upsweep<false>, dim3(ntiles), dim3(kRadixThreads), 0, st,
+                               src_k, n, shift, mask, ntiles, hist, pm, c, scenes,
+                               (uint32_t*)nullptr, err_flag);
+        if (m <= kRadixLocalScanMax) {
+            hipLaunchKernelGGL(k_radix_downsweep<true>, dim3(ntiles), dim3(kRadixThreads), 0, st,
+                               src_k, src_v, dst_k, dst_v, n, shift, mask, b, ntiles, hist);
+        } else {
+            hipError_t se = scan_exclusive(hist, m, scan_scratch, st);
+            if (se != hipSuccess) return se;
+            hipLaunchKernelGGL(k_radix_downsweep<false>, dim3(ntiles), dim3(kRadixThreads), 0, st,
+                               src_k, src_v, dst_k, dst_v, n, shift, mask, b, ntiles, hist);
+        }
         uint32_t* nk = (dst_k == keys_alt) ? keys : keys_alt;
         uint32_t* nv = (dst_v == vals) ? vals_alt : vals;
         src_k = dst_k;

@@ -23,6 +23,7 @@
 // (dx, dy, dz, id) order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include "sphk_walk_common.h"
@@ -61,18 +62,6 @@ __global__ __launch_bounds__(256) void k_chunk_fill(const uint32_t* __restrict__
     const uint32_t a = cs[first], b = cs[first + (uint32_t)g.nz];
     uint32_t o = offs[col];
     for (uint32_t s = a; s < b; s += 64u) items[o++] = s;
-}
-
-hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, const SceneGrid* scenes,
-                         uint32_t* cnt_scratch, uint32_t* scan_scratch, uint32_t* items,
-                         uint32_t* nitems, hipStream_t st) {
-    hipLaunchKernelGGL(k_chunk_count, dim3((c.ncols + 1 + 255) / 256), dim3(256), 0, st,
-                       cell_start, c, scenes, cnt_scratch);
-    hipError_t e = scan_exclusive(cnt_scratch, c.ncols + 1, scan_scratch, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_chunk_fill, dim3((c.ncols + 255) / 256 + 1), dim3(256), 0, st, cell_start,
-                       c, scenes, cnt_scratch, items, nitems);
-    return hipGetLastError();
 }
 
 uint32_t max_chunks(uint32_t n, const StepConsts& c) { return (n + 63u) / 64u + c.ncols; }
@@ -953,36 +942,125 @@ __global__ __launch_bounds__(256) void k_force_finish(
 }
 
 // Particles with a non-finite coordinate (dense index G) have no neighbour, not even
-// themselves: rho = m W(0), a = (0, g, 0); they are integrated and clamped like the rest.
-// The step's last kernel: it also advances the simulation time (time_dev, if given) by dt.
-__global__ __launch_bounds__(256) void k_nonfinite_step(
-    const float4* __restrict__ spm, float4* __restrict__ svr, const uint32_t* __restrict__ sid,
-    const uint32_t* __restrict__ cs, uint32_t n, StepConsts c,
-    const SceneGrid* __restrict__ scenes, const float* __restrict__ dt_dev,
-    float dt_arg, float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc_out,
-    float* __restrict__ rho_out, float* __restrict__ prs_out, float* __restrict__ time_dev) {
-    if (time_dev && blockIdx.x == 0 && threadIdx.x == 0)
-        *time_dev = *time_dev + (dt_dev ? *dt_dev : dt_arg);
-    const uint32_t first = cs[c.G];
-    const uint32_t s = first + blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const float4 pi = spm[s];
-    const float4 vi = svr[s];
-    const float rho_i = pi.w * c.w0;
-    const float P_i = c.k_gas * (rho_i - c.rho0);
-    const float3 a = final_accel(make_float3(0.f, 0.f, 0.f), make_float3(0.f, 0.f, 0.f), pi.w,
-                                 rho_i, c);
-    const float dt = dt_dev ? *dt_dev : dt_arg;
-    float3 v = make_float3(vi.x, vi.y, vi.z);
-    float4 p = pi;
-    const uint32_t id = sid[s];
-    const SceneGrid& g = scenes[scene_of_particle(scenes, c.nscenes, id)];
-    integrate_clamp(p, v, a, dt, c, g.lo, g.hi);
-    pm[id] = p;
-    vel[id] = make_float4(v.x, v.y, v.z, 0.0f);
-    acc_out[id] = make_float4(a.x, a.y, a.z, 0.0f);
-    rho_out[id] = rho_i;
-    prs_out[id] = P_i;
+// themselves: rho = m W(0), a = (0, g, 0); they are integrated and clamped like the rest.  The
+// bin holds the sorted slots [cs[G], n); workgroup `b` of `nb` takes every nb-th run of 256.
+// Workgroup 0 also advances the simulation time (time_dev, if given) by dt.
+__device__ __forceinline__ void nonfinite_particles(const StepTail& t,
+                                                    const uint32_t* __restrict__ cs,
+                                                    const StepConsts& c,
+                                                    const SceneGrid* __restrict__ scenes,
+                                                    uint32_t b, uint32_t nb) {
+    const float dt = t.dt_dev ? *t.dt_dev : t.dt;
+    if (t.time_dev && b == 0 && threadIdx.x == 0) *t.time_dev = *t.time_dev + dt;
+    for (uint32_t s = cs[c.G] + b * blockDim.x + threadIdx.x; s < t.n; s += nb * blockDim.x) {
+        const float4 pi = t.spm[s];
+        const float4 vi = t.svr[s];
+        const float rho_i = pi.w * c.w0;
+        const float P_i = c.k_gas * (rho_i - c.rho0);
+        const float3 a = final_accel(make_float3(0.f, 0.f, 0.f), make_float3(0.f, 0.f, 0.f),
+                                     pi.w, rho_i, c);
+        float3 v = make_float3(vi.x, vi.y, vi.z);
+        float4 p = pi;
+        const uint32_t id = t.sid[s];
+        const SceneGrid& g = scenes[scene_of_particle(scenes, c.nscenes, id)];
+        integrate_clamp(p, v, a, dt, c, g.lo, g.hi);
+        t.pm[id] = p;
+        t.vel[id] = make_float4(v.x, v.y, v.z, 0.0f);
+        t.acc[id] = make_float4(a.x, a.y, a.z, 0.0f);
+        t.rho[id] = rho_i;
+        t.prs[id] = P_i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nonfinite_step(StepTail t, const uint32_t* __restrict__ cs,
+                                                        StepConsts c,
+                                                        const SceneGrid* __restrict__ scenes) {
+    nonfinite_particles(t, cs, c, scenes, blockIdx.x, gridDim.x);
+}
+
+// Grids of at most this many (x, y) columns get their chunk list from one launch: every
+// workgroup counts the chunks of all columns and scans the counts in LDS, then writes the
+// items of its own 256 columns.  Workgroups from `fill_blocks` on do the non-finite bin's step
+// (kTail).
+constexpr uint32_t kChunkListLocal = 8192;
+
+template <bool kTail>
+__global__ __launch_bounds__(256) void k_chunk_list(const uint32_t* __restrict__ cs, StepConsts c,
+                                                    const SceneGrid* __restrict__ scenes,
+                                                    uint32_t* __restrict__ items,
+                                                    uint32_t* __restrict__ nitems,
+                                                    uint32_t fill_blocks, StepTail t) {
+    if (blockIdx.x >= fill_blocks) {
+        if constexpr (kTail)
+            nonfinite_particles(t, cs, c, scenes, blockIdx.x - fill_blocks,
+                                gridDim.x - fill_blocks);
+        return;
+    }
+    constexpr int kPer = (int)(kChunkListLocal / 256u);
+    __shared__ uint32_t off[kChunkListLocal];
+    __shared__ uint32_t lw[4];
+    const uint32_t ncols = c.ncols;
+    // thread k counts the columns [k*per, (k+1)*per), all its loads in flight at once
+    const uint32_t per = (ncols + 255u) / 256u;
+    const uint32_t beg = threadIdx.x * per;
+    uint32_t cnt[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t col = beg + (uint32_t)j;
+        cnt[j] = 0;
+        if ((uint32_t)j < per && col < ncols) {
+            const SceneGrid& g = scenes[scene_of_col(scenes, c.nscenes, col)];
+            const uint32_t first = g.cell_base + (col - g.col_base) * (uint32_t)g.nz;
+            cnt[j] = (cs[first + (uint32_t)g.nz] - cs[first] + 63u) / 64u;
+        }
+        sum += cnt[j];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan_256(sum, lw, &total);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t col = beg + (uint32_t)j;
+        if ((uint32_t)j < per && col < ncols) off[col] = run;
+        run += cnt[j];
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) nitems[0] = total;
+    const uint32_t col = blockIdx.x * 256u + threadIdx.x;
+    if (col >= ncols) return;
+    const SceneGrid& g = scenes[scene_of_col(scenes, c.nscenes, col)];
+    const uint32_t first = g.cell_base + (col - g.col_base) * (uint32_t)g.nz;
+    const uint32_t a = cs[first], b = cs[first + (uint32_t)g.nz];
+    uint32_t o = off[col];
+    for (uint32_t s = a; s < b; s += 64u) items[o++] = s;
+}
+
+hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, const SceneGrid* scenes,
+                         uint32_t* cnt_scratch, uint32_t* scan_scratch, uint32_t* items,
+                         uint32_t* nitems, const StepTail* tail, hipStream_t st) {
+    // the non-finite bin holds at most n particles; its start is read on the device
+    const uint32_t tail_blocks =
+        tail ? std::max<uint32_t>(1u, std::min<uint32_t>((tail->n + 255u) / 256u, 64u)) : 0u;
+    if (c.ncols <= kChunkListLocal) {
+        const uint32_t fill_blocks = std::max<uint32_t>(1u, (c.ncols + 255u) / 256u);
+        if (tail)
+            hipLaunchKernelGGL(k_chunk_list<true>, dim3(fill_blocks + tail_blocks), dim3(256), 0,
+                               st, cell_start, c, scenes, items, nitems, fill_blocks, *tail);
+        else
+            hipLaunchKernelGGL(k_chunk_list<false>, dim3(fill_blocks), dim3(256), 0, st,
+                               cell_start, c, scenes, items, nitems, fill_blocks, StepTail{});
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_chunk_count, dim3((c.ncols + 1 + 255) / 256), dim3(256), 0, st,
+                       cell_start, c, scenes, cnt_scratch);
+    hipError_t e = scan_exclusive(cnt_scratch, c.ncols + 1, scan_scratch, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_chunk_fill, dim3((c.ncols + 255) / 256 + 1), dim3(256), 0, st, cell_start,
+                       c, scenes, cnt_scratch, items, nitems);
+    if (tail)
+        hipLaunchKernelGGL(k_nonfinite_step, dim3(tail_blocks), dim3(256), 0, st, *tail,
+                           cell_start, c, scenes);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -1137,8 +1215,8 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
                              const uint32_t* cell_start, uint32_t n, const StepConsts& c,
                              const SceneGrid* scenes, const float* dt_dev, float dt, float4* pm,
                              float4* vel, float4* acc, float* rho, float* prs, int shape,
-                             float* time_dev, unsigned long long* nbr_stats, hipStream_t st) {
-    if (n == 0) return time_dev ? launch_advance_time(time_dev, dt_dev, dt, st) : hipSuccess;
+                             unsigned long long* nbr_stats, hipStream_t st) {
+    if (n == 0) return hipSuccess;
     PartSums ps;
     if (hipError_t err = part_sums(wb, n, shape, &ps)) return err;
     const uint32_t chunks = max_chunks(n, c);
@@ -1160,10 +1238,6 @@ hipError_t launch_force_walk(const WalkBuffers& wb, const float4* spm, const uin
         hipLaunchKernelGGL(k_force_finish, dim3((n + 255) / 256), dim3(256), 0, st, spm, svr,
                            sid, cell_start, c, scenes, ps, dt_dev, dt, pm, vel, acc, rho, prs,
                            nbr_stats);
-    // the non-finite bin holds at most n particles; the kernel reads its start on the device
-    hipLaunchKernelGGL(k_nonfinite_step, dim3((n + 255) / 256), dim3(256), 0, st, spm,
-                       const_cast<float4*>(svr), sid, cell_start, n, c, scenes, dt_dev, dt, pm, vel,
-                       acc, rho, prs, time_dev);
     return hipGetLastError();
 }
 
