@@ -121,6 +121,10 @@ struct sphk_engine {
     std::vector<SceneGrid> table;  // host copy of the device scene table
     SceneGrid* d_scenes = nullptr;
     size_t d_scenes_cap = 0;
+    // what d_scenes holds: the table last uploaded and the buffer generation it went to (a
+    // restore that leaves the grids as they were re-uploads nothing)
+    std::vector<SceneGrid> table_uploaded;
+    uint64_t table_uploaded_gen = ~0ull;
     StepConsts consts{};
     bool sorted_valid = false;  // work arrays hold the cell-sorted image of the current state
     uint64_t buffer_gen = 0;    // bumped when a buffer used by the step is reallocated

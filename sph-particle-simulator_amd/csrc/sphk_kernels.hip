@@ -186,14 +186,18 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_downsweep(
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
     uint32_t mask, int bits, uint32_t ntiles, const uint32_t* __restrict__ offs) {
     __shared__ uint32_t wc[kRadixWaves << kRadixMaxBits];
-    __shared__ uint32_t loffs[kLocalScan ? (1 << kRadixMaxBits) : 1];
     const uint32_t nb = mask + 1;
     for (uint32_t i = threadIdx.x; i < kRadixWaves * nb; i += kRadixThreads) wc[i] = 0;
+    // the tile's digit offsets (kLocalScan; declared in its branch so that the other kernel
+    // keeps exactly 32 KB of LDS, five workgroups per CU)
+    const uint32_t* loffs = nullptr;
     if constexpr (kLocalScan) {
         // thread t sums the words [t*per, (t+1)*per) of the digit-major histogram; the
         // exclusive prefix of the block gives each word's global prefix, kept for the words of
         // this tile (positions d * ntiles + blockIdx.x)
         __shared__ uint32_t lsw[kRadixWaves];
+        __shared__ uint32_t lo[1 << kRadixMaxBits];
+        loffs = lo;
         const uint32_t m = nb * ntiles;
         const uint32_t per = (m + kRadixThreads - 1) / kRadixThreads;
         const uint32_t beg = threadIdx.x * per;
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_downsweep(
                                           : blockIdx.x + ((beg - blockIdx.x + ntiles - 1) / ntiles) * ntiles;
         for (uint32_t i = beg; i < end; ++i) {
             if (i == next) {
-                loffs[i / ntiles] = run;
+                lo[i / ntiles] = run;
                 next += ntiles;
             }
             run += offs[i];
