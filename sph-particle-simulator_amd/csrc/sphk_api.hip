@@ -421,6 +421,11 @@ int ensure_grid(sphk_engine* e) {
                       scan_scratch_words(std::max<size_t>(hist_words, ncols + 1)), e->buffer_gen))
         return rc;
     e->scan_scratch_cap = sscap;
+    uint64_t lcap = e->lookback_cap;
+    if (int rc = grow(&e->lookback, lcap, lookback_state_words(hist_words), e->buffer_gen))
+        return rc;
+    if (lcap != e->lookback_cap) HIP_TRY(hipMemsetAsync(e->lookback, 0, lcap * 4, e->stream));
+    e->lookback_cap = lcap;
     e->grid_valid = true;
     return SPHK_OK;
 }
@@ -440,7 +445,7 @@ int build_sorted(sphk_engine* e, const StepTail* tail) {
     uint32_t* ko = nullptr;
     uint32_t* vo = nullptr;
     HIP_TRY(radix_sort_keys(e->pm, e->consts, e->d_scenes, e->uscal, e->keys, e->keys_alt, e->vals,
-                            e->vals_alt, e->hist, e->scan_scratch, n,
+                            e->vals_alt, e->hist, e->lookback, e->scan_scratch, n,
                             total_key_bits(e->consts.G), &ko, &vo, st));
     e->skey = ko;
     e->sid = vo;
@@ -461,11 +466,13 @@ int build_sorted(sphk_engine* e, const StepTail* tail) {
 }
 
 // A finite particle outside the dense grid is routed to the non-finite bin and
-// flagged; the grid sizing above makes this unreachable, so report it loudly.
+// flagged (bit 0); the grid sizing above makes this unreachable, so report it loudly.  Bit 1:
+// a look-back of the sort's scan gave up waiting (its result is not to be trusted).
 int check_grid_flag(sphk_engine* e) {
     uint32_t flag = 0;
     HIP_TRY(hipMemcpyAsync(&flag, e->uscal, 4, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    if (flag & 2u) return set_error(SPHK_E_DOMAIN, "the cell sort's scan timed out waiting");
     if (flag) return set_error(SPHK_E_DOMAIN, "a finite particle fell outside the cell grid");
     return SPHK_OK;
 }
@@ -782,7 +789,8 @@ int sphk_destroy(sphk_engine* e) {
     for (void* p : {(void*)e->walk.items, (void*)e->walk.nitems, (void*)e->walk.colcnt,
                     (void*)e->walk.mask, (void*)e->walk.wj0, (void*)e->walk.nwords,
                     (void*)e->walk.pstarts, (void*)e->walk.part_acc, (void*)e->walk.part_cnt,
-                    (void*)e->walk.part_force, (void*)e->scan_scratch, (void*)e->d_scenes})
+                    (void*)e->walk.part_force, (void*)e->scan_scratch, (void*)e->d_scenes,
+                    (void*)e->lookback})
         if (p) (void)hipFree(p);
     if (e->sur) surrogate_free(e->sur);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);

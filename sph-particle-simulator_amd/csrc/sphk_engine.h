@@ -91,6 +91,8 @@ struct sphk_engine {
     uint32_t* hist = nullptr;
     uint32_t* scan_scratch = nullptr;
     size_t scan_scratch_cap = 0;
+    uint32_t* lookback = nullptr;  // state of the radix histograms' one-launch scan (zeroed)
+    size_t lookback_cap = 0;
     WalkBuffers walk;
     int shape_density = 0;  // walk launch shapes (walk_shape_name), set by sphk_set_walk_shapes
     int shape_force = 0;

@@ -197,13 +197,18 @@ struct StepTail {
 // launchers; all enqueue on `st`
 size_t scan_scratch_words(size_t m);
 hipError_t scan_exclusive(uint32_t* data, uint32_t m, uint32_t* scratch, hipStream_t st);
+// one-launch scan; `state` (lookback_state_words(m) words) is zero before and after each call;
+// a look-back that waits past its bound sets bit 1 of *err
+size_t lookback_state_words(size_t m);
+hipError_t scan_exclusive_lookback(uint32_t* data, uint32_t m, uint32_t* state, uint32_t* err,
+                                   hipStream_t st);
 // stable LSD sort of the particles' dense cell indices (computed from pm in the first pass and
 // left in `keys`) with their caller ids as values
 hipError_t radix_sort_keys(const float4* pm, const StepConsts& c, const SceneGrid* scenes,
                            uint32_t* err_flag, uint32_t* keys, uint32_t* keys_alt, uint32_t* vals,
-                           uint32_t* vals_alt, uint32_t* hist, uint32_t* scan_scratch, uint32_t n,
-                           int total_bits, uint32_t** keys_out, uint32_t** vals_out,
-                           hipStream_t st);
+                           uint32_t* vals_alt, uint32_t* hist, uint32_t* lookback,
+                           uint32_t* scan_scratch, uint32_t n, int total_bits,
+                           uint32_t** keys_out, uint32_t** vals_out, hipStream_t st);
 size_t radix_hist_words(uint32_t n, int total_bits);
 hipError_t launch_cell_start(const uint32_t* skey, uint32_t n, uint32_t G, uint32_t* cell_start,
                              hipStream_t st);

@@ -149,6 +149,118 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(uint32_t* __restric
     }
 }
 
+// Single-launch exclusive scan of uint32 with decoupled look-back.  `state` holds a ticket
+// counter, a completion counter and one status word per tile, all zero between calls.  Each
+// workgroup takes the next ticket as its tile (tickets follow start order), publishes its tile
+// sum (kAggregate), then wavefront 0 reads the status words of the 64 tiles before it and adds
+// them back to the nearest inclusive prefix (kInclusive), moving 64 tiles further back while none
+// is in sight, and publishes its own inclusive prefix.  A tile waits only for tiles that took
+// their tickets before it and publish their sums without waiting on anything, so every wait
+// ends; the wait is also bounded (kLookbackSpins, then *err is set and the scan is wrong but
+// finishes).  The last workgroup to finish clears the counters and status words.  Values are
+// below 2^30 (sums of at most 2^27 particles).
+constexpr uint32_t kAggregate = 1u << 30;
+constexpr uint32_t kInclusive = 2u << 30;
+constexpr uint32_t kStatusValue = (1u << 30) - 1u;
+constexpr uint32_t kLookbackSpins = 1u << 22;
+constexpr uint32_t kLookbackMaxTiles = 64;
+
+__device__ __forceinline__ uint32_t status_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_lookback(uint32_t* __restrict__ data,
+                                                                uint32_t m, uint32_t ntiles,
+                                                                uint32_t* __restrict__ state,
+                                                                uint32_t* __restrict__ err) {
+    __shared__ uint32_t lw[kScanThreads / 64];
+    __shared__ uint32_t s_tile, s_prefix, s_last;
+    uint32_t* status = state + 2;
+    if (threadIdx.x == 0) s_tile = atomicAdd(&state[0], 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * kScanTile + threadIdx.x * kScanPer;
+    uint32_t v[kScanPer];
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const uint32_t i = base + k;
+        v[k] = i < m ? data[i] : 0u;
+        acc += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan_256(acc, lw, &tot);
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x;
+        if (lane == 0) status_store(&status[tile], (tile == 0 ? kInclusive : kAggregate) | tot);
+        uint32_t prefix = 0;
+        int32_t hi = (int32_t)tile - 1;  // the window is tiles hi, hi - 1, ..., hi - 63
+        uint32_t spins = 0;
+        while (hi >= 0) {
+            const int32_t j = hi - (int32_t)lane;
+            // tiles before the first count as an inclusive prefix of zero
+            const uint32_t w = j >= 0 ? status_load(&status[j]) : kInclusive;
+            const uint64_t incl = __ballot((w & kInclusive) != 0u);
+            const uint64_t wait = __ballot(w == 0u);
+            // lanes up to the nearest inclusive prefix, or all 64 if none is in the window
+            const uint64_t need = incl ? (~0ull >> (63 - __builtin_ctzll(incl))) : ~0ull;
+            if (wait & need) {
+                if (++spins >= kLookbackSpins) {
+                    if (lane == 0) atomicOr(err, 2u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            uint32_t part = ((need >> lane) & 1ull) ? (w & kStatusValue) : 0u;
+            for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+            prefix += part;
+            if (incl) break;
+            hi -= 64;
+        }
+        if (lane == 0) {
+            if (tile > 0) status_store(&status[tile], kInclusive | ((prefix + tot) & kStatusValue));
+            s_prefix = prefix;
+        }
+    }
+    __syncthreads();
+    run += s_prefix;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const uint32_t i = base + k;
+        if (i < m) data[i] = run;
+        run += v[k];
+    }
+    // completion: the last workgroup clears the state for the next call (every other workgroup
+    // is through its look-back when it counts itself done)
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(&state[1], 1u) == ntiles - 1u;
+    }
+    __syncthreads();
+    if (s_last) {
+        for (uint32_t t = threadIdx.x; t < ntiles; t += kScanThreads) status_store(&status[t], 0u);
+        if (threadIdx.x == 0) {
+            status_store(&state[0], 0u);
+            status_store(&state[1], 0u);
+        }
+    }
+}
+
+size_t lookback_state_words(size_t m) { return 2 + (m + kScanTile - 1) / kScanTile; }
+
+hipError_t scan_exclusive_lookback(uint32_t* data, uint32_t m, uint32_t* state, uint32_t* err,
+                                   hipStream_t st) {
+    if (m == 0) return hipSuccess;
+    const uint32_t tiles = (m + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_lookback, dim3(tiles), dim3(kScanThreads), 0, st, data, m, tiles,
+                       state, err);
+    return hipGetLastError();
+}
+
 size_t scan_scratch_words(size_t m) {
     size_t words = 0;
     while (m > kScanTile) {
@@ -282,9 +394,9 @@ size_t radix_hist_words(uint32_t n, int total_bits) {
 // to `keys`.
 hipError_t radix_sort_keys(const float4* pm, const StepConsts& c, const SceneGrid* scenes,
                            uint32_t* err_flag, uint32_t* keys, uint32_t* keys_alt, uint32_t* vals,
-                           uint32_t* vals_alt, uint32_t* hist, uint32_t* scan_scratch, uint32_t n,
-                           int total_bits, uint32_t** keys_out, uint32_t** vals_out,
-                           hipStream_t st) {
+                           uint32_t* vals_alt, uint32_t* hist, uint32_t* lookback,
+                           uint32_t* scan_scratch, uint32_t n, int total_bits,
+                           uint32_t** keys_out, uint32_t** vals_out, hipStream_t st) {
     int passes, bits_per;
     radix_plan(total_bits, passes, bits_per);
     if (total_bits < 1) total_bits = 1;
@@ -310,7 +422,11 @@ hipError_t radix_sort_keys(const float4* pm, const StepConsts& c, const SceneGri
             hipLaunchKernelGGL(k_radix_downsweep<true>, dim3(ntiles), dim3(kRadixThreads), 0, st,
                                src_k, src_v, dst_k, dst_v, n, shift, mask, b, ntiles, hist);
         } else {
-            hipError_t se = scan_exclusive(hist, m, scan_scratch, st);
+            // one launch while few tiles poll the status words; beyond that their polling
+            // costs more than the three launches (9.6M particles, 587 tiles: +50 us per pass)
+            hipError_t se = m <= kLookbackMaxTiles * kScanTile
+                                ? scan_exclusive_lookback(hist, m, lookback, err_flag, st)
+                                : scan_exclusive(hist, m, scan_scratch, st);
             if (se != hipSuccess) return se;
             hipLaunchKernelGGL(k_radix_downsweep<false>, dim3(ntiles), dim3(kRadixThreads), 0, st,
                                src_k, src_v, dst_k, dst_v, n, shift, mask, b, ntiles, hist);

@@ -88,6 +88,69 @@ def test_sorted_order_matches_hash_buckets(sphk, oracle, scenes):
         assert g.tolist() == buckets[k]
 
 
+def _scan_paths(n, bits):
+    """Which scan each radix pass uses for n keys of `bits` bits (tiles of 2,048 keys): inside
+    the downsweep (at most 8,192 histogram words), one look-back launch (at most 64 scan tiles
+    of 4,096 words) or three launches."""
+    passes = -(-bits // 11)
+    per = -(-bits // passes)
+    tiles = -(-n // 2048)
+    out = []
+    for p in range(passes):
+        m = (1 << min(per, bits - p * per)) * tiles
+        out.append("local" if m <= 8192 else "lookback" if m <= 64 * 4096 else "three")
+    return out
+
+
+@pytest.mark.parametrize("n", [20_000, 300_000, 1_200_000])
+def test_sorted_order_every_scan_path(sphk, oracle, scenes, n):
+    # the cell-sorted order is the stable order by (cell x, y, z, caller id) whichever scan the
+    # radix passes take, and stays so over repeated builds (the look-back scan's state is
+    # cleared by each launch for the next)
+    s = scenes.config1()
+    rng = np.random.default_rng(n)
+    lo = np.array([0.0, 0.0, 0.0]) + 0.01
+    hi = np.array([3.0, 1.2, 1.5]) - 0.01
+    pos = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+    e = sphk.Engine(n, s.params(sphk))
+    coords, _ = oracle.cell_keys(pos, np.float32(0.04))
+    want = np.lexsort((np.arange(n), coords[:, 2], coords[:, 1], coords[:, 0]))
+    dims = coords.max(0) - coords.min(0) + 5
+    paths = _scan_paths(n, int(np.prod(dims.astype(np.int64))).bit_length())
+    print("scan paths", n, paths, flush=True)
+    for rep in range(3):
+        e.upload(pos)
+        order, cells = e.sorted_order()
+        np.testing.assert_array_equal(order, want.astype(np.uint32))
+        assert np.all(np.diff(cells.astype(np.int64)) >= 0)
+
+
+def test_graph_replay_matches_eager_mid_size(sphk, scenes):
+    # 300,000 particles: the radix passes scan with the one-launch look-back, whose state the
+    # replays must find cleared
+    s = scenes.config1()
+    rng = np.random.default_rng(4)
+    pos = rng.uniform([0.05, 0.05, 0.05], [2.95, 1.15, 1.45], size=(300_000, 3)).astype(np.float32)
+    outs = []
+    for graph in (False, True):
+        e = sphk.Engine(len(pos), s.params(sphk, 1))
+        e.set_graph(graph)
+        e.upload(pos)
+        e.snapshot_save()
+        res = []
+        for k in range(3):
+            e.snapshot_restore()
+            e.step(1e-4)
+            res.append((e.positions(), e.densities()))
+        for k in range(3):
+            e.step(1e-4)
+        res.append((e.positions(), e.velocities()))
+        outs.append(res)
+    for a, b in zip(outs[0], outs[1]):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+
+
 def test_neighbors_density_accel_step0_config0(sphk, oracle, scenes):
     s = scenes.config0()
     e = _engine(sphk, s)
