@@ -984,6 +984,19 @@ __global__ __launch_bounds__(256) void k_nonfinite_step(StepTail t, const uint32
 // (kTail).
 constexpr uint32_t kChunkListLocal = 8192;
 
+// Sorted slots [x, y) of column `col` (its cells are consecutive, z fastest).  kOne: a
+// one-scene grid, whose geometry g0 is loaded once instead of looked up per column.
+template <bool kOne>
+__device__ __forceinline__ uint2 column_slots(const uint32_t* __restrict__ cs,
+                                              const SceneGrid* __restrict__ scenes,
+                                              const StepConsts& c, const SceneGrid& g0,
+                                              uint32_t col) {
+    const SceneGrid& g = kOne ? g0 : scenes[scene_of_col(scenes, c.nscenes, col)];
+    const uint32_t nz = (uint32_t)g.nz;
+    const uint32_t first = g.cell_base + (col - g.col_base) * nz;
+    return make_uint2(cs[first], cs[first + nz]);
+}
+
 template <bool kTail>
 __global__ __launch_bounds__(256) void k_chunk_list(const uint32_t* __restrict__ cs, StepConsts c,
                                                     const SceneGrid* __restrict__ scenes,
@@ -1003,18 +1016,30 @@ __global__ __launch_bounds__(256) void k_chunk_list(const uint32_t* __restrict__
     // thread k counts the columns [k*per, (k+1)*per), all its loads in flight at once
     const uint32_t per = (ncols + 255u) / 256u;
     const uint32_t beg = threadIdx.x * per;
+    const SceneGrid g0 = scenes[0];
+    const bool one = c.nscenes == 1;
     uint32_t cnt[kPer];
     uint32_t sum = 0;
+    if (one) {
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        const uint32_t col = beg + (uint32_t)j;
-        cnt[j] = 0;
-        if ((uint32_t)j < per && col < ncols) {
-            const SceneGrid& g = scenes[scene_of_col(scenes, c.nscenes, col)];
-            const uint32_t first = g.cell_base + (col - g.col_base) * (uint32_t)g.nz;
-            cnt[j] = (cs[first + (uint32_t)g.nz] - cs[first] + 63u) / 64u;
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t col = beg + (uint32_t)j;
+            const uint2 r = (uint32_t)j < per && col < ncols
+                                ? column_slots<true>(cs, scenes, c, g0, col)
+                                : make_uint2(0u, 0u);
+            cnt[j] = (r.y - r.x + 63u) / 64u;
+            sum += cnt[j];
         }
-        sum += cnt[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t col = beg + (uint32_t)j;
+            const uint2 r = (uint32_t)j < per && col < ncols
+                                ? column_slots<false>(cs, scenes, c, g0, col)
+                                : make_uint2(0u, 0u);
+            cnt[j] = (r.y - r.x + 63u) / 64u;
+            sum += cnt[j];
+        }
     }
     uint32_t total;
     uint32_t run = block_exclusive_scan_256(sum, lw, &total);
@@ -1028,11 +1053,10 @@ __global__ __launch_bounds__(256) void k_chunk_list(const uint32_t* __restrict__
     if (blockIdx.x == 0 && threadIdx.x == 0) nitems[0] = total;
     const uint32_t col = blockIdx.x * 256u + threadIdx.x;
     if (col >= ncols) return;
-    const SceneGrid& g = scenes[scene_of_col(scenes, c.nscenes, col)];
-    const uint32_t first = g.cell_base + (col - g.col_base) * (uint32_t)g.nz;
-    const uint32_t a = cs[first], b = cs[first + (uint32_t)g.nz];
+    const uint2 r = one ? column_slots<true>(cs, scenes, c, g0, col)
+                        : column_slots<false>(cs, scenes, c, g0, col);
     uint32_t o = off[col];
-    for (uint32_t s = a; s < b; s += 64u) items[o++] = s;
+    for (uint32_t s = r.x; s < r.y; s += 64u) items[o++] = s;
 }
 
 hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, const SceneGrid* scenes,
