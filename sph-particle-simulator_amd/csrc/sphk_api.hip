@@ -449,8 +449,8 @@ int build_sorted(sphk_engine* e, const StepTail* tail) {
                             total_key_bits(e->consts.G), &ko, &vo, st));
     e->skey = ko;
     e->sid = vo;
-    HIP_TRY(launch_cell_start(e->skey, n, e->consts.G, e->cell_start, st));
-    HIP_TRY(launch_gather(e->pm, e->vel, e->sid, n, e->consts.qscale, e->sq, e->spm, e->svr, st));
+    HIP_TRY(launch_gather_cells(e->pm, e->vel, e->sid, e->skey, n, e->consts.G, e->consts.qscale,
+                                e->sq, e->spm, e->svr, e->cell_start, st));
     StepTail t;
     if (tail) {
         t = *tail;

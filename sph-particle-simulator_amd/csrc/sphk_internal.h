@@ -210,11 +210,11 @@ hipError_t radix_sort_keys(const float4* pm, const StepConsts& c, const SceneGri
                            uint32_t* scan_scratch, uint32_t n, int total_bits,
                            uint32_t** keys_out, uint32_t** vals_out, hipStream_t st);
 size_t radix_hist_words(uint32_t n, int total_bits);
-hipError_t launch_cell_start(const uint32_t* skey, uint32_t n, uint32_t G, uint32_t* cell_start,
-                             hipStream_t st);
-hipError_t launch_gather(const float4* pm, const float4* vel, const uint32_t* sid, uint32_t n,
-                         double qscale, uint2* sq,
-                         float4* spm, float4* svr, hipStream_t st);
+// sorted copies (spm, svr, sq) and the cell table cell_start[0, G + 2), one launch
+hipError_t launch_gather_cells(const float4* pm, const float4* vel, const uint32_t* sid,
+                               const uint32_t* skey, uint32_t n, uint32_t G, double qscale,
+                               uint2* sq, float4* spm, float4* svr, uint32_t* cell_start,
+                               hipStream_t st);
 uint32_t max_chunks(uint32_t n, const StepConsts& c);
 // chunk list of the sorted particles; with `tail`, also the non-finite particles' step
 hipError_t launch_chunks(const uint32_t* cell_start, const StepConsts& c, const SceneGrid* scenes,

@@ -3,9 +3,9 @@
 // Step pipeline (one HIP stream):
 //   radix sort     stable LSD sort of (cell, caller id) -> cell-sorted order; the first pass
 //                  computes the dense cell index of every particle (caller order)
-//   cell_start     occupancy prefix of the dense grid (binary search per cell)
-//   gather         cell-sorted float4 copies of (x,y,z,m) and (vx,vy,vz,-), quantized
-//                  coordinates of the candidate filter
+//   gather+cells   cell-sorted float4 copies of (x,y,z,m) and (vx,vy,vz,-), quantized
+//                  coordinates of the candidate filter; beside them (same launch) the
+//                  occupancy prefix of the dense grid (binary search per cell)
 //   chunks         64-particle work items per (x,y) column of the grid
 //   density, force walks   (sphk_walk.hip)
 #include <hip/hip_runtime.h>
@@ -25,12 +25,14 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // ---------------------------------------------------------------------------
 // Dense index of particle i at p: its cell in its scene's grid, or G (the non-finite bin) for a
 // non-finite coordinate.  A finite particle outside the grid's interior sets *err.
+// kOne: a one-scene grid, whose geometry g0 is loaded once instead of looked up per particle.
+template <bool kOne>
 __device__ __forceinline__ uint32_t cell_key(const float4& p, uint32_t i, const StepConsts& c,
                                              const SceneGrid* __restrict__ scenes,
-                                             uint32_t* __restrict__ err) {
+                                             const SceneGrid& g0, uint32_t* __restrict__ err) {
     uint32_t L = c.G;
     if (__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z)) {
-        const SceneGrid g = scenes[scene_of_particle(scenes, c.nscenes, i)];
+        const SceneGrid& g = kOne ? g0 : scenes[scene_of_particle(scenes, c.nscenes, i)];
         float fx = floorf(p.x * c.inv_cell);
         float fy = floorf(p.y * c.inv_cell);
         float fz = floorf(p.z * c.inv_cell);
@@ -82,12 +84,24 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_upsweep(
             const uint32_t idx = base + k * kRadixThreads + threadIdx.x;
             p[k] = idx < n ? pm[idx] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        const SceneGrid g0 = scenes[0];
+        if (c.nscenes == 1) {
 #pragma unroll
-        for (int k = 0; k < kRadixItems; ++k) {
-            const uint32_t idx = base + k * kRadixThreads + threadIdx.x;
-            if (idx < n) {
-                key[k] = cell_key(p[k], idx, c, scenes, err);
-                keys_out[idx] = key[k];
+            for (int k = 0; k < kRadixItems; ++k) {
+                const uint32_t idx = base + k * kRadixThreads + threadIdx.x;
+                if (idx < n) {
+                    key[k] = cell_key<true>(p[k], idx, c, scenes, g0, err);
+                    keys_out[idx] = key[k];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kRadixItems; ++k) {
+                const uint32_t idx = base + k * kRadixThreads + threadIdx.x;
+                if (idx < n) {
+                    key[k] = cell_key<false>(p[k], idx, c, scenes, g0, err);
+                    keys_out[idx] = key[k];
+                }
             }
         }
     } else {
@@ -444,32 +458,6 @@ hipError_t radix_sort_keys(const float4* pm, const StepConsts& c, const SceneGri
 }
 
 // ---------------------------------------------------------------------------
-// cell_start[c] = first sorted slot with key >= c, for c in [0, G+1]
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_cell_start(const uint32_t* __restrict__ skey, uint32_t n,
-                                                    uint32_t ncs, uint32_t* __restrict__ cs) {
-    uint32_t cidx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cidx >= ncs) return;
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (skey[mid] < cidx)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    cs[cidx] = lo;
-}
-
-hipError_t launch_cell_start(const uint32_t* skey, uint32_t n, uint32_t G, uint32_t* cell_start,
-                             hipStream_t st) {
-    uint32_t ncs = G + 2;
-    hipLaunchKernelGGL(k_cell_start, dim3((ncs + 255) / 256), dim3(256), 0, st, skey, n, ncs,
-                       cell_start);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
 // gather into cell-sorted order
 // ---------------------------------------------------------------------------
 // Quantized coordinate of the candidate filter: floor(x * qscale) modulo 2^32, computed in
@@ -481,17 +469,39 @@ __device__ __forceinline__ uint32_t quantize(float x, double qscale) {
     return (uint32_t)(v - 4294967296.0 * floor(v * (1.0 / 4294967296.0)));
 }
 
-__global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ pm,
-                                                const float4* __restrict__ vel,
-                                                const uint32_t* __restrict__ sid, uint32_t n,
-                                                double qscale, uint2* __restrict__ sq,
-                                                float4* __restrict__ spm, float4* __restrict__ svr) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+// One launch, two independent parts that need only the sorted keys and ids: workgroups below
+// `gather_blocks` gather particle s into sorted order, the rest fill the cell table
+// cs[c] = first sorted slot with key >= c, for c in [0, G + 1] (binary search per cell).
+__global__ __launch_bounds__(256) void k_gather_cells(const float4* __restrict__ pm,
+                                                      const float4* __restrict__ vel,
+                                                      const uint32_t* __restrict__ sid,
+                                                      const uint32_t* __restrict__ skey,
+                                                      uint32_t n, double qscale,
+                                                      uint2* __restrict__ sq,
+                                                      float4* __restrict__ spm,
+                                                      float4* __restrict__ svr,
+                                                      uint32_t gather_blocks, uint32_t ncs,
+                                                      uint32_t* __restrict__ cs) {
+    if (blockIdx.x >= gather_blocks) {
+        const uint32_t cidx = (blockIdx.x - gather_blocks) * blockDim.x + threadIdx.x;
+        if (cidx >= ncs) return;
+        uint32_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (skey[mid] < cidx)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        cs[cidx] = lo;
+        return;
+    }
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    uint32_t id = sid[s];
+    const uint32_t id = sid[s];
     const float4 p = pm[id];
     spm[s] = p;
-    float4 v = vel[id];
+    const float4 v = vel[id];
     svr[s] = make_float4(v.x, v.y, v.z, 0.0f);
     // x and y keep their low 16 bits (packed), z its low 32: differences of neighbours are
     // exact in 16- and 24-bit wrap-around arithmetic
@@ -499,11 +509,15 @@ __global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ pm,
     sq[s] = make_uint2((qx & 0xffffu) | (qy << 16), quantize(p.z, qscale));
 }
 
-hipError_t launch_gather(const float4* pm, const float4* vel, const uint32_t* sid, uint32_t n,
-                         double qscale, uint2* sq, float4* spm, float4* svr, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, pm, vel, sid, n, qscale,
-                       sq, spm, svr);
+hipError_t launch_gather_cells(const float4* pm, const float4* vel, const uint32_t* sid,
+                               const uint32_t* skey, uint32_t n, uint32_t G, double qscale,
+                               uint2* sq, float4* spm, float4* svr, uint32_t* cell_start,
+                               hipStream_t st) {
+    const uint32_t gather_blocks = (n + 255) / 256;
+    const uint32_t ncs = G + 2;
+    const uint32_t cell_blocks = (ncs + 255) / 256;
+    hipLaunchKernelGGL(k_gather_cells, dim3(gather_blocks + cell_blocks), dim3(256), 0, st, pm,
+                       vel, sid, skey, n, qscale, sq, spm, svr, gather_blocks, ncs, cell_start);
     return hipGetLastError();
 }
 
