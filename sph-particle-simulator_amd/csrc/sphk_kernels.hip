@@ -349,12 +349,28 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_downsweep(
     const uint32_t base = blockIdx.x * kRadixTile + wave * (kRadixItems * 64);
     uint32_t* mywc = wc + wave * nb;
     uint32_t key[kRadixItems];
+    uint32_t val[kRadixItems];
+    uint32_t base_off[kRadixItems];
     uint32_t rank[kRadixItems];
+    // keys, values and (unless scanned here) the digits' global offsets are all requested
+    // before the ranking, which hides their latency
+#pragma unroll
+    for (int k = 0; k < kRadixItems; ++k) {
+        const uint32_t idx = base + k * 64 + lane;
+        key[k] = idx < n ? keys_in[idx] : 0u;
+        val[k] = vals_in ? (idx < n ? vals_in[idx] : 0u) : idx;
+    }
+    if constexpr (!kLocalScan) {
+#pragma unroll
+        for (int k = 0; k < kRadixItems; ++k) {
+            const uint32_t idx = base + k * 64 + lane;
+            base_off[k] = idx < n ? offs[((key[k] >> shift) & mask) * ntiles + blockIdx.x] : 0u;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < kRadixItems; ++k) {
         uint32_t idx = base + k * 64 + lane;
         bool valid = idx < n;
-        key[k] = valid ? keys_in[idx] : 0u;
         uint32_t d = (key[k] >> shift) & mask;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < bits; ++b) {
@@ -383,9 +399,9 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_downsweep(
         uint32_t idx = base + k * 64 + lane;
         if (idx < n) {
             uint32_t d = (key[k] >> shift) & mask;
-            uint32_t dest = (kLocalScan ? loffs[d] : offs[d * ntiles + blockIdx.x]) + mywc[d] + rank[k];
+            uint32_t dest = (kLocalScan ? loffs[d] : base_off[k]) + mywc[d] + rank[k];
             keys_out[dest] = key[k];
-            vals_out[dest] = vals_in ? vals_in[idx] : idx;
+            vals_out[dest] = val[k];
         }
     }
 }
