@@ -196,57 +196,66 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(uint32_t* __rest
     if (threadIdx.x == 0) s_tile = atomicAdd(&state[0], 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint32_t base = tile * kScanTile + threadIdx.x * kScanPer;
-    uint32_t v[kScanPer];
-    uint32_t acc = 0;
+    if (tile < ntiles) {
+        const uint32_t base = tile * kScanTile + threadIdx.x * kScanPer;
+        uint32_t v[kScanPer];
+        uint32_t acc = 0;
 #pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-        const uint32_t i = base + k;
-        v[k] = i < m ? data[i] : 0u;
-        acc += v[k];
-    }
-    uint32_t tot;
-    uint32_t run = block_exclusive_scan_256(acc, lw, &tot);
-    if (threadIdx.x < 64) {
-        const uint32_t lane = threadIdx.x;
-        if (lane == 0) status_store(&status[tile], (tile == 0 ? kInclusive : kAggregate) | tot);
-        uint32_t prefix = 0;
-        int32_t hi = (int32_t)tile - 1;  // the window is tiles hi, hi - 1, ..., hi - 63
-        uint32_t spins = 0;
-        while (hi >= 0) {
-            const int32_t j = hi - (int32_t)lane;
-            // tiles before the first count as an inclusive prefix of zero
-            const uint32_t w = j >= 0 ? status_load(&status[j]) : kInclusive;
-            const uint64_t incl = __ballot((w & kInclusive) != 0u);
-            const uint64_t wait = __ballot(w == 0u);
-            // lanes up to the nearest inclusive prefix, or all 64 if none is in the window
-            const uint64_t need = incl ? (~0ull >> (63 - __builtin_ctzll(incl))) : ~0ull;
-            if (wait & need) {
-                if (++spins >= kLookbackSpins) {
-                    if (lane == 0) atomicOr(err, 2u);
-                    break;
+        for (int k = 0; k < kScanPer; ++k) {
+            const uint32_t i = base + k;
+            v[k] = i < m ? data[i] : 0u;
+            acc += v[k];
+        }
+        uint32_t tot;
+        uint32_t run = block_exclusive_scan_256(acc, lw, &tot);
+        if (threadIdx.x < 64) {
+            const uint32_t lane = threadIdx.x;
+            if (lane == 0)
+                status_store(&status[tile], (tile == 0 ? kInclusive : kAggregate) | tot);
+            uint32_t prefix = 0;
+            int32_t hi = (int32_t)tile - 1;  // the window is tiles hi, hi - 1, ..., hi - 63
+            uint32_t spins = 0;
+            while (hi >= 0) {
+                const int32_t j = hi - (int32_t)lane;
+                // tiles before the first count as an inclusive prefix of zero
+                const uint32_t w = j >= 0 ? status_load(&status[j]) : kInclusive;
+                const uint64_t incl = __ballot((w & kInclusive) != 0u);
+                const uint64_t wait = __ballot(w == 0u);
+                // lanes up to the nearest inclusive prefix, or all 64 if none is in the window
+                const uint64_t need = incl ? (~0ull >> (63 - __builtin_ctzll(incl))) : ~0ull;
+                if (wait & need) {
+                    if (++spins >= kLookbackSpins) {
+                        if (lane == 0) atomicOr(err, 2u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
                 }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+                uint32_t part = ((need >> lane) & 1ull) ? (w & kStatusValue) : 0u;
+                for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+                prefix += part;
+                if (incl) break;
+                hi -= 64;
             }
-            uint32_t part = ((need >> lane) & 1ull) ? (w & kStatusValue) : 0u;
-            for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
-            prefix += part;
-            if (incl) break;
-            hi -= 64;
+            if (lane == 0) {
+                if (tile > 0)
+                    status_store(&status[tile], kInclusive | ((prefix + tot) & kStatusValue));
+                s_prefix = prefix;
+            }
         }
-        if (lane == 0) {
-            if (tile > 0) status_store(&status[tile], kInclusive | ((prefix + tot) & kStatusValue));
-            s_prefix = prefix;
-        }
-    }
-    __syncthreads();
-    run += s_prefix;
+        __syncthreads();
+        run += s_prefix;
 #pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-        const uint32_t i = base + k;
-        if (i < m) data[i] = run;
-        run += v[k];
+        for (int k = 0; k < kScanPer; ++k) {
+            const uint32_t i = base + k;
+            if (i < m) data[i] = run;
+            run += v[k];
+        }
+    } else if (threadIdx.x == 0) {
+        // a ticket counter left non-zero (it is cleared at allocation and by every call's last
+        // workgroup): flag it and touch no data; tiles waiting on a missing ticket give up at
+        // their bound, and the call still clears the state below
+        atomicOr(err, 2u);
     }
     // completion: the last workgroup clears the state for the next call (every other workgroup
     // is through its look-back when it counts itself done)
