@@ -199,14 +199,17 @@ int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t 
  * (the matrix-core filter's output), produced for the current state with the engine's
  * density launch shape; nothing but work arrays is written.  Per chunk (64 consecutive
  * sorted slots of one grid column): items[c] its first sorted slot, nwords[c] its words,
- * wj0[c * 128 + w] the first sorted slot of word w, mask[(c * 128 + w) * 64 + l] lane l's
- * bits (candidate k in bit 31 - k).  Chunks of more than 128 words store only their first
- * 128 (the force walk tests such chunks again).  Call with items == NULL to get the chunk
- * count in *nchunks, then with arrays of that many chunks.  Replaces no reference
- * function: it exposes the filter that stands in for SpatialHash::query_squared
+ * wj0[c * 256 + w] the first sorted slot of word w, mask[(c * 256 + w) * 64 + l] lane l's
+ * bits (candidate k in bit 31 - k).  The engine stores W words per chunk (*words_stored,
+ * if given: 256, or 128 where the word buffers would pass 16 GiB); chunks of more than W
+ * words store only their first W (the force walk tests such chunks again), and words W to
+ * 255 of every row are left as they were.  Call with items == NULL to get the chunk count
+ * in *nchunks, then with arrays of that many chunks.  Replaces no reference function: it
+ * exposes the filter that stands in for SpatialHash::query_squared
  * (src/spatial_hash.cpp:31-57) so its pairs can be checked one by one. */
 int sphk_debug_walk_words(sphk_engine* e, uint32_t* items, uint32_t* nwords, uint32_t* wj0,
-                          uint32_t* mask, size_t chunk_cap, size_t* nchunks);
+                          uint32_t* mask, size_t chunk_cap, size_t* nchunks,
+                          size_t* words_stored);
 /* Grid shape of the last build: origin cell (3), dims (3), number of cells. */
 int sphk_debug_grid(sphk_engine* e, int32_t* origin3, int32_t* dims3, uint64_t* ncells);
 

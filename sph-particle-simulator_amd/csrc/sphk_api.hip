@@ -378,15 +378,21 @@ int ensure_grid(sphk_engine* e) {
             *p = nullptr;
         }
         const size_t want = std::max<size_t>(chunks, wb.chunk_cap * 5 / 4);
+        // kMaxWords words (64 KiB of bits) per chunk while the buffers stay within the budget
+        const uint32_t mw =
+            (uint64_t)want * kMaxWords * 64 * 4 <= kWordBufferBudget ? kMaxWords : kMinWords;
         wb.chunk_cap = 0;
+        wb.max_words = 0;
         HIP_TRY(dalloc(&wb.items, want));
-        HIP_TRY(dalloc(&wb.mask, want * kMaxWords * 64));
-        HIP_TRY(dalloc(&wb.wj0, want * kMaxWords));
+        HIP_TRY(dalloc(&wb.mask, want * mw * 64));
+        HIP_TRY(dalloc(&wb.wj0, want * mw));
         HIP_TRY(dalloc(&wb.nwords, want));
         HIP_TRY(dalloc(&wb.pstarts, want));
         wb.chunk_cap = want;
+        wb.max_words = mw;
         ++e->buffer_gen;
     }
+    e->consts.max_words = wb.max_words;
     if (!e->shapes_explicit) {
         const bool small = e->n < kSplitBelow;
         e->shape_density = small ? kShapeSmall : kShapeLargeDensity;
@@ -1368,7 +1374,8 @@ int sphk_debug_neighbors(sphk_engine* e, int64_t* offsets, int64_t* idx, size_t 
 }
 
 int sphk_debug_walk_words(sphk_engine* e, uint32_t* items, uint32_t* nwords, uint32_t* wj0,
-                          uint32_t* mask, size_t chunk_cap, size_t* nchunks) {
+                          uint32_t* mask, size_t chunk_cap, size_t* nchunks,
+                          size_t* words_stored) {
     if (int rc = check_engine(e)) return rc;
     if (!nchunks) return set_error(SPHK_E_INVALID, "null chunk count");
     *nchunks = 0;
@@ -1381,15 +1388,21 @@ int sphk_debug_walk_words(sphk_engine* e, uint32_t* items, uint32_t* nwords, uin
     HIP_TRY(hipMemcpyAsync(&nc, e->walk.nitems, 4, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     *nchunks = nc;
+    const size_t mw = e->walk.max_words;
+    if (words_stored) *words_stored = mw;
     if (!items) return SPHK_OK;
     if (chunk_cap < nc) return set_error(SPHK_E_INVALID, "need %u chunks", nc);
     if (!nwords || !wj0 || !mask) return set_error(SPHK_E_INVALID, "null output array");
-    const size_t words = (size_t)nc * kMaxWords;
     HIP_TRY(hipMemcpyAsync(items, e->walk.items, (size_t)nc * 4, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipMemcpyAsync(nwords, e->walk.nwords, (size_t)nc * 4, hipMemcpyDeviceToHost,
                            e->stream));
-    HIP_TRY(hipMemcpyAsync(wj0, e->walk.wj0, words * 4, hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipMemcpyAsync(mask, e->walk.mask, words * 64 * 4, hipMemcpyDeviceToHost, e->stream));
+    // the caller's rows hold kMaxWords words per chunk; the engine's hold mw
+    if (nc) {
+        HIP_TRY(hipMemcpy2DAsync(wj0, kMaxWords * 4, e->walk.wj0, mw * 4, mw * 4, nc,
+                                 hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpy2DAsync(mask, kMaxWords * 64 * 4, e->walk.mask, mw * 64 * 4, mw * 64 * 4,
+                                 nc, hipMemcpyDeviceToHost, e->stream));
+    }
     HIP_TRY(hipStreamSynchronize(e->stream));
     return SPHK_OK;
 }

@@ -42,6 +42,8 @@ struct StepConsts {
     uint32_t pad_slot;  // sorted slot past every particle (index = capacity): zero mass and
                         // weights, far away; the walks read it in place of a neighbour
     int32_t qneg;       // -(T + 1), T the threshold of the quantized candidate filter
+    uint32_t max_words; // candidate words stored per chunk (the word buffers' stride); chunks
+                        // with more are tested again by the force walk
     double qscale;      // quantized coordinate = floor(x * qscale): 256 units per cell
 };
 
@@ -79,8 +81,13 @@ constexpr int kRadixMaxBits = 11;
 // of the walks' 32-bit row offsets.
 constexpr uint64_t kMaxCapacity = (1ull << 27) - 2;
 
-// neighbour walks: words of 32 candidates kept in LDS per wave, and per chunk in HBM
-constexpr int kMaxWords = 128;
+// neighbour walks: candidate words (32 candidates each) stored per chunk in HBM for the force
+// walk: kMaxWords, or kMinWords where the word buffers would pass kWordBufferBudget bytes.
+// Chunks next to a free surface can reach ~160 words (a sparse column beside full ones);
+// one the force walk must test again outlasts the rest of its launch.
+constexpr int kMaxWords = 256;
+constexpr int kMinWords = 128;
+constexpr uint64_t kWordBufferBudget = 16ull << 30;
 constexpr int kNumWalkShapes = 9;  // launch shapes selectable per walk (see walk_shape_name)
 const char* walk_shape_name(int v);
 // Parts of a chunk's candidate columns (corners, edges, centre with the outer ring).
@@ -102,11 +109,12 @@ struct WalkBuffers {
     uint32_t* items = nullptr;    // first sorted slot of each chunk
     uint32_t* nitems = nullptr;   // [1] number of chunks
     uint32_t* colcnt = nullptr;   // [ncols + 1] per-column chunk counts -> offsets
-    uint32_t* mask = nullptr;     // [chunks][kMaxWords][64] neighbour bits
-    uint32_t* wj0 = nullptr;      // [chunks][kMaxWords] first slot of each word
+    uint32_t* mask = nullptr;     // [chunks][max_words][64] neighbour bits
+    uint32_t* wj0 = nullptr;      // [chunks][max_words] first slot of each word
     uint32_t* nwords = nullptr;   // [chunks] words of each chunk
-    uint32_t* pstarts = nullptr;  // [chunks] first stored word of parts 1 and 2, 8 bits each
+    uint32_t* pstarts = nullptr;  // [chunks] first stored word of parts 1 and 2, 16 bits each
     size_t chunk_cap = 0;
+    uint32_t max_words = 0;       // stride of mask and wj0 (kMaxWords or kMinWords)
     size_t col_cap = 0;
     // per-part sums of the split shapes, [3][part_cap] by sorted slot (allocated only
     // while a split shape is selected)

@@ -18,7 +18,8 @@
 //   those pairs (pairs the filter let through beyond the radius weigh exactly zero).
 // The density pass stores the words of each chunk in HBM (masks, wj0, word count, part
 // starts) and the force rows of each particle; the force pass reads them back instead of
-// testing again.  Chunks with more than kMaxWords words (pile-ups of thousands of particles
+// testing again.  Chunks with more words than the engine stores (StepConsts::max_words: 256,
+// or 128 for engines whose word buffers would pass 16 GiB; pile-ups of thousands of particles
 // per cell) are re-tested by the force pass.  The neighbour-list export walks the reference's
 // (dx, dy, dz, id) order.
 #include <hip/hip_runtime.h>
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
     WordTable wt;
     const bool tab = word_table_build(wt, ch, cs, c, part, lane);
     if (lane == 0 && part <= 0) {
-        gpart[item] = (uint32_t)wt.ps1 | (uint32_t)wt.ps2 << 8;
+        gpart[item] = pack_parts(wt.ps1, wt.ps2);
         gnw[item] = (uint32_t)wt.total;
     }
     Cursor cu;
@@ -292,7 +293,8 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
                     more = false;
                 } else {
                     wave_sync();
-                    store_words(item, seg_base, nw, lm_new, lj_new, gmask, gwj0, lane);
+                    store_words(item, c.max_words, seg_base, nw, lm_new, lj_new, gmask, gwj0,
+                                lane);
                     seg_base += nw;
                     if (tab ? ti >= wt.n : cu.done) more = false;
                     if (walk_part < 0 || seg_part == walk_part) {
@@ -357,7 +359,7 @@ __global__ __launch_bounds__(Shape::threads) __attribute__((amdgpu_waves_per_eu(
                 }
             }
             wave_sync();
-            store_words(item, seg_base, nw, lmask, lwj0, gmask, gwj0, lane);
+            store_words(item, c.max_words, seg_base, nw, lmask, lwj0, gmask, gwj0, lane);
             BitWalk<kSeg> bw;
             bw.begin(sl.nonempty, lmask, lwj0, lane);
             const uint32_t steps = wave_max(sl.count);
@@ -713,7 +715,7 @@ void k_force_walk(
     uint32_t cnt = 0;  // neighbours of this wavefront's part(s)
     int cur_part = 0;
     const int total_words = (int)uni(gnw[item]);
-    const bool stored = total_words <= kMaxWords;
+    const bool stored = total_words <= (int)c.max_words;
     const uint32_t pstarts = stored ? uni(gpart[item]) : 0u;
     Cursor cu;
     if (!stored) cursor_begin(cu, ch, cs, c, 0, part);
@@ -752,7 +754,7 @@ void k_force_walk(
                         seg_part = part_of_word(seg_base, pstarts);
                         const int pend_end = part_word_end(pstarts, seg_part, total_words);
                         nw = min(kSeg - 1, pend_end - seg_base);
-                        const size_t base = (size_t)item * kMaxWords + seg_base;
+                        const size_t base = (size_t)item * c.max_words + seg_base;
                         uint32_t wv[kSeg - 1];
 #pragma unroll
                         for (int w = 0; w < kSeg - 1; ++w)
@@ -825,7 +827,7 @@ void k_force_walk(
                 seg_part = part_of_word(seg_base, pstarts);
                 const int pend = part_word_end(pstarts, seg_part, total_words);
                 nw = min(kSeg - 1, pend - seg_base);
-                const size_t base = (size_t)item * kMaxWords + seg_base;
+                const size_t base = (size_t)item * c.max_words + seg_base;
                 sl.nonempty = 0;
                 sl.count = 0;
                 // all of the segment's word loads are issued before the first is used (a loop

@@ -292,15 +292,13 @@ __device__ __forceinline__ bool cursor_settle(Cursor& cu, const Chunk& ch,
     return !cu.done;
 }
 
-// Part starts of a chunk's stored words, packed 8 bits each: words before part 1, part 2.
-__device__ __forceinline__ uint32_t pack_parts(const Cursor& cu) {
-    const uint32_t b1 = (uint32_t)cu.pw0;
-    const uint32_t b2 = b1 + (uint32_t)cu.pw1;
-    return b1 | b2 << 8;
+// Part starts of a chunk's stored words, packed 16 bits each (words before part 1, part 2).
+__device__ __forceinline__ uint32_t pack_parts(int ps1, int ps2) {
+    return (uint32_t)ps1 | (uint32_t)ps2 << 16;
 }
 // First stored word of part p, and the end of its words.
 __device__ __forceinline__ int part_word_start(uint32_t parts, int p) {
-    return p == 0 ? 0 : (int)((parts >> (8 * (p - 1))) & 0xffu);
+    return p == 0 ? 0 : (int)((parts >> (16 * (p - 1))) & 0xffffu);
 }
 __device__ __forceinline__ int part_word_end(uint32_t parts, int p, int total) {
     return p == 2 ? total : part_word_start(parts, p + 1);
@@ -779,20 +777,21 @@ __device__ __forceinline__ void add_nbr_stats(uint32_t cnt, unsigned long long* 
 
 // Stores the LDS words [0, nw) of segment `seg_base` of this chunk into HBM (nontemporal: the
 // force walk reads them once, long after, and the rows the walks gather should keep L2).
-__device__ __forceinline__ void store_words(uint32_t item, int seg_base, int nw,
+__device__ __forceinline__ void store_words(uint32_t item, uint32_t max_words, int seg_base,
+                                            int nw,
                                             const uint32_t* __restrict__ lmask,
                                             const uint32_t* __restrict__ lwj0,
                                             uint32_t* __restrict__ gmask,
                                             uint32_t* __restrict__ gwj0, uint32_t lane) {
-    const size_t base = (size_t)item * kMaxWords;
+    const size_t base = (size_t)item * max_words;
     for (int w = 0; w < nw; ++w) {
         int gw = seg_base + w;
-        if (gw >= kMaxWords) break;
+        if (gw >= (int)max_words) break;
         __builtin_nontemporal_store(lmask[w * 64 + lane], &gmask[(base + gw) * 64 + lane]);
     }
     for (int w = (int)lane; w < nw; w += 64) {
         int gw = seg_base + w;
-        if (gw < kMaxWords) gwj0[base + gw] = lwj0[w];
+        if (gw < (int)max_words) gwj0[base + gw] = lwj0[w];
     }
 }
 
@@ -921,7 +920,7 @@ __device__ __forceinline__ bool build_list(ListBuilder& lb, bool stored, int tot
     lb.cnt = 0u;
     bool any = false;
     if (stored) {
-        const size_t base = (size_t)item * kMaxWords;
+        const size_t base = (size_t)item * c.max_words;
         // a window of up to kListCap words cannot overflow a column list
         const int end = min(total_words, lb.wpos + kListCap);
         while (lb.wpos < end) {
@@ -1006,7 +1005,7 @@ __device__ __forceinline__ void walk_pairs(uint32_t item, const Chunk& ch, uint3
                                            Body&& body) {
     const uint32_t col = lane & 31u, p = lane >> 5;
     const int total_words = (int)uni(wb.nwords[item]);
-    const bool stored = total_words <= kMaxWords;
+    const bool stored = total_words <= (int)c.max_words;
     const uint32_t s_safe = active ? s : ch.s0;
     Cursor cu;
     if (!stored) cursor_begin(cu, ch, cs_, c);

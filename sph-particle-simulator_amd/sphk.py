@@ -122,7 +122,7 @@ SIGNATURES = {
     "sphk_debug_sorted": [_P, _U32, _U32, _SZ],
     "sphk_debug_neighbors": [_P, _I64, _I64, _SZ, C.POINTER(_SZ)],
     "sphk_debug_grid": [_P, _I32, _I32, _U64],
-    "sphk_debug_walk_words": [_P, _U32, _U32, _U32, _U32, _SZ, C.POINTER(_SZ)],
+    "sphk_debug_walk_words": [_P, _U32, _U32, _U32, _U32, _SZ, C.POINTER(_SZ), C.POINTER(_SZ)],
     "sphk_set_scenes": [_P, _SZ, _U64, _F],
     "sphk_scene_count": [_P, C.POINTER(_SZ)],
     "sphk_set_graph": [_P, C.c_int],
@@ -385,18 +385,22 @@ class Engine:
 
     def walk_words(self):
         """The production density walk's stored candidate words for the current state:
-        (items, nwords, wj0 [chunks, 128], mask [chunks, 128, 64])."""
+        (items, nwords, wj0 [chunks, W], mask [chunks, W, 64]), W the words the engine stores
+        per chunk (chunks with more words store their first W)."""
         nc = _SZ()
-        check(self.lib.sphk_debug_walk_words(self.h, None, None, None, None, 0, C.byref(nc)))
+        stored = _SZ()
+        check(self.lib.sphk_debug_walk_words(self.h, None, None, None, None, 0, C.byref(nc),
+                                             C.byref(stored)))
         c = nc.value
         items = np.empty(max(c, 1), dtype=np.uint32)
         nw = np.empty(max(c, 1), dtype=np.uint32)
-        wj0 = np.empty((max(c, 1), 128), dtype=np.uint32)
-        mask = np.empty((max(c, 1), 128, 64), dtype=np.uint32)
+        wj0 = np.zeros((max(c, 1), 256), dtype=np.uint32)
+        mask = np.zeros((max(c, 1), 256, 64), dtype=np.uint32)
         check(self.lib.sphk_debug_walk_words(self.h, _ptr(items, C.c_uint32), _ptr(nw, C.c_uint32),
                                              _ptr(wj0, C.c_uint32), _ptr(mask, C.c_uint32),
-                                             c, C.byref(nc)))
-        return items[:c], nw[:c], wj0[:c], mask[:c]
+                                             c, C.byref(nc), None))
+        w = stored.value
+        return items[:c], nw[:c], wj0[:c, :w], mask[:c, :w]
 
     def grid(self):
         o = np.empty(3, dtype=np.int32)

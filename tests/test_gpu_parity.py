@@ -295,7 +295,7 @@ def _stored_pairs(e, n):
         lanes = np.arange(a, min(a + 64, n))
         lanes = lanes[(cells[lanes] // nz) == col]
         nw = int(nwords[c])
-        if nw > 128:
+        if nw > wj0.shape[1]:  # more words than the engine stores: tested again
             overflow += 1
             continue
         bits = ((mask[c, :nw, :len(lanes), None] >> (31 - k)) & 1).astype(bool)  # [w, l, k]
@@ -580,11 +580,11 @@ def test_walk_shapes_bitexact(sphk, scenes):
 
 @pytest.mark.parametrize("split", [False, True, "pp"])
 def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes, split):
-    # a pile-up cell with more than 4096 candidates per chunk exceeds the stored-word budget;
-    # the force pass then re-tests.  Both paths must agree with the oracle, in the
-    # single-wavefront and the three-wavefront (split) walks.
+    # a pile-up whose chunks have more candidate words than the engine stores (256 words of
+    # 32); the force pass then re-tests them.  Both paths must agree with the oracle, in the
+    # single-wavefront, the three-wavefront (split) and the ping-pong walks.
     rng = np.random.default_rng(3)
-    pos = np.concatenate([rng.uniform(0.1, 0.13, size=(3000, 3)),
+    pos = np.concatenate([rng.uniform(0.1, 0.13, size=(10000, 3)),
                           rng.uniform(0.3, 0.5, size=(500, 3))]).astype(np.float32)
     p = sphk.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
     e = sphk.Engine(len(pos), p)
@@ -595,6 +595,8 @@ def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes, split):
         v = names.index("seg16_waves4_split")
         e.set_walk_shapes(v, v)
     e.upload(pos)
+    _, nwords, wj0, _ = e.walk_words()
+    assert nwords.max() > wj0.shape[1], "no chunk takes the re-test path"
     o = oracle.Oracle(len(pos), oracle.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0,
                                                       zmax=1))
     o.set_particles(pos)
@@ -612,15 +614,19 @@ def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes, split):
 
 
 def test_pileup_split_and_single_walks_bitexact(sphk):
-    # Pile-up chunks have more than 128 candidate words: the single-wavefront density walk
-    # steps its cursor through them, the split walk enumerates each part's words in a word
-    # table, and the force walk re-tests them.  Every route walks the same words in the same
-    # order, so all shapes give the same bits.
+    # Pile-up chunks have more candidate words than the engine stores: the single-wavefront
+    # density walk steps its cursor through them, the split walk enumerates each part's words
+    # in a word table, and the force walk re-tests them.  Every route walks the same words in
+    # the same order, so all shapes give the same bits.
     rng = np.random.default_rng(11)
-    pos = np.concatenate([rng.uniform(0.1, 0.13, size=(3000, 3)),
+    pos = np.concatenate([rng.uniform(0.1, 0.13, size=(10000, 3)),
                           rng.uniform(0.3, 0.5, size=(2000, 3))]).astype(np.float32)
     names = sphk.walk_shape_names()
     outs = []
+    e = sphk.Engine(len(pos), sphk.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1))
+    e.upload(pos)
+    _, nwords, wj0, _ = e.walk_words()
+    assert nwords.max() > wj0.shape[1], "no chunk takes the re-test path"
     for d, f in ((0, 0), (names.index("seg16_waves4_split"), names.index("seg32_waves4_split")),
                  (1, 0), (names.index("pp12_waves4"), names.index("pp16_waves4")),
                  (names.index("pp16_waves4"), names.index("pp12_waves4"))):
