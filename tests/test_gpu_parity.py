@@ -613,6 +613,34 @@ def test_force_retest_path_matches_stored_masks(sphk, oracle, scenes, split):
     assert err.max() <= ACC_TOL
 
 
+def test_word_store_fallback_bitexact(sphk):
+    # An engine whose word buffers would pass 16 GiB (capacity 17M: 265,000 chunk slots) stores
+    # 128 candidate words per chunk instead of 256; chunks of 129-256 words are then tested
+    # again by the force walk instead of read back.  Both routes walk the same words in the
+    # same order: the two engines give the same bits.
+    rng = np.random.default_rng(5)
+    pos = np.concatenate([rng.uniform(0.1, 0.13, size=(6000, 3)),
+                          rng.uniform(0.3, 0.5, size=(1000, 3))]).astype(np.float32)
+    p = sphk.default_params(xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
+    outs = []
+    for cap in (len(pos), 17_000_000):
+        e = sphk.Engine(cap, p)
+        e.upload(pos)
+        _, nwords, wj0, _ = e.walk_words()
+        stored = wj0.shape[1]
+        assert stored == (256 if cap == len(pos) else 128)
+        if cap == len(pos):
+            # chunks that this engine reads back and the other tests again
+            assert np.any((nwords > 128) & (nwords <= 256))
+        for _ in range(2):
+            e.step(1e-4)
+        outs.append((e.positions()[: len(pos)], e.velocities()[: len(pos)],
+                     e.densities()[: len(pos)]))
+        e.close()
+    for a, b in zip(outs[0], outs[1]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_pileup_split_and_single_walks_bitexact(sphk):
     # Pile-up chunks have more candidate words than the engine stores: the single-wavefront
     # density walk steps its cursor through them, the split walk enumerates each part's words
