@@ -98,7 +98,27 @@ static bucket* hm_find(const hashmap* m, uint64_t key) {
     }
 }
 
+/* Doubles the table, moving every bucket (its items keep their order). */
+static void hm_grow(hashmap* m) {
+    const size_t ns = m->nslots * 2;
+    bucket* slots = (bucket*)calloc(ns, sizeof(bucket));
+    const size_t mask = ns - 1;
+    for (size_t k = 0; k < m->nslots; ++k) {
+        const bucket* b = &m->slots[k];
+        if (!b->used) continue;
+        size_t i = (size_t)mix64(b->key) & mask;
+        while (slots[i].used) i = (i + 1) & mask;
+        slots[i] = *b;
+    }
+    free(m->slots);
+    m->slots = slots;
+    m->nslots = ns;
+}
+
+/* The table is kept at most half full (a sparse scene has about as many cells as particles),
+ * so a probe always reaches a free slot. */
 static bucket* hm_get_or_insert(hashmap* m, uint64_t key) {
+    if (2 * (m->count + 1) > m->nslots) hm_grow(m);
     size_t mask = m->nslots - 1;
     size_t i = (size_t)mix64(key) & mask;
     for (;;) {

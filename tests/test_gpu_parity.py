@@ -391,6 +391,42 @@ def test_nonfinite_particles(sphk, oracle):
     np.testing.assert_array_equal(coords_g, coords_o)
 
 
+def test_wide_grid_chunk_path(sphk, oracle):
+    # A grid of more than 8,192 (x, y) columns (here 105 x 105) lists its chunks in three
+    # launches (counts, their scan, fill) and steps the non-finite bin's particles and the
+    # clock in a launch of its own; the results match the oracle as on the one-launch path.
+    rng = np.random.default_rng(21)
+    pos = rng.uniform([0.05, 0.05, 0.05], [3.95, 3.95, 0.35], size=(60000, 3)).astype(np.float32)
+    pos[7] = [np.nan, 0.1, 0.1]
+    pos[11] = [0.2, np.inf, 0.3]
+    bounds = dict(xmin=0, xmax=4, ymin=0, ymax=4, zmin=0, zmax=0.4)
+    e = sphk.Engine(len(pos), sphk.default_params(**bounds))
+    e.upload(pos)
+    _, dims, _ = e.grid()
+    assert int(dims[0]) * int(dims[1]) > 8192
+    o = oracle.Oracle(len(pos), oracle.default_params(**bounds))
+    o.set_particles(pos)
+    _oracle_phases(o)
+    offs, idx = _check_neighbors(e, o)
+    e.step(0.001)
+    assert e.time() == np.float32(0.001)
+    rho_o = o.get(2)
+    fin = np.isfinite(pos).all(axis=1)
+    rel = np.abs(e.densities()[: len(pos)].astype(np.float64) - rho_o) / np.abs(rho_o)
+    assert rel.max() <= RHO_TOL
+    acc = e.accelerations()
+    for i in (7, 11):
+        np.testing.assert_array_equal(acc[i], np.array([0, -9.81, 0], np.float32))
+    mass = np.full(len(pos), 0.001, np.float32)
+    scale = oracle.accel_term_scale(pos, np.zeros_like(pos), mass, rho_o, offs, idx, o.params)
+    # particles alone in their radius feel gravity only, exactly
+    alone = fin & (scale == 0)
+    np.testing.assert_array_equal(acc[alone].astype(np.float64), o.get(4)[alone])
+    some = fin & (scale > 0)
+    err = np.linalg.norm(acc[some].astype(np.float64) - o.get(4)[some], axis=1) / scale[some]
+    assert err.max() <= ACC_TOL
+
+
 def test_consistent_formulation_multistep(sphk, oracle, scenes):
     # the opt-in consistent formulation stays bounded at dt = 1e-4: 20 steps track the oracle
     s = scenes.small_block(10, 10, 10, seed=2)

@@ -154,6 +154,19 @@ def test_neighbors_equal_brute_force(oracle):
     assert all((i, i) in pairs for i in range(len(pos)))
 
 
+def test_sparse_scene_neighbors_equal_brute_force(oracle):
+    # about as many occupied cells as particles (the cell table must grow past its first
+    # size, set for dense scenes): same neighbour sets as brute force
+    rng = np.random.default_rng(9)
+    pos = rng.uniform(-2.0, 2.0, size=(4000, 3)).astype(np.float32)
+    pos[: 400] = rng.uniform(0.0, 0.06, size=(400, 3)).astype(np.float32)  # a dense corner
+    o = _oracle_step0(oracle, pos)
+    offs, idx = o.neighbors()
+    boffs, bidx = oracle.brute_neighbors(pos, R2)
+    for i in range(len(pos)):
+        assert sorted(idx[offs[i]:offs[i + 1]]) == list(bidx[boffs[i]:boffs[i + 1]])
+
+
 def test_neighbor_order_is_cell_lexicographic(oracle):
     rng = np.random.default_rng(5)
     pos = rng.uniform(0, 0.16, size=(600, 3)).astype(np.float32)
