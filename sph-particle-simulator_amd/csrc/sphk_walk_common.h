@@ -555,10 +555,12 @@ __device__ __forceinline__ uint32_t mfma_filter_word(MfmaFilter& mf,
         x = __builtin_amdgcn_alignbit(x, __float_as_uint(d1[v]), 31);
         y = __builtin_amdgcn_alignbit(y, __float_as_uint(d2[v]), 31);
     }
-    // lanes 0-31 own targets 0-31 (candidates 0-15 here, 16-31 in lane + 32's x); lanes 32-63
-    // own targets 32-63 (candidates 16-31 here, 0-15 in lane - 32's y)
-    const uint32_t recv = (uint32_t)__shfl_xor((int)(lo ? y : x), 32);
-    const uint32_t word = lo ? (x << 16 | recv) : (recv << 16 | y);
+    // lanes 0-31 own targets 0-31 (candidates 0-15 in x, 16-31 in lane + 32's x); lanes 32-63
+    // own targets 32-63 (candidates 0-15 in lane - 32's y, 16-31 in y).  v_permlane32_swap
+    // trades the upper lanes' x for the lower lanes' y, after which every lane holds its
+    // target's candidates 0-15 in x and 16-31 in y.
+    const auto xy = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+    const uint32_t word = xy[0] << 16 | xy[1];
     return word & (~0u << (32u - cnt));
 }
 
