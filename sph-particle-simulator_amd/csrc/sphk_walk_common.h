@@ -536,9 +536,8 @@ __device__ __forceinline__ uint32_t mfma_filter_word(MfmaFilter& mf,
     a[5] = (_Float16)1.0f;
     a[6] = (_Float16)2048.0f;
     a[7] = (_Float16)0.0f;
-    const bool lo = lane < 32u;
-    const sphk_half8 zero = {};
-    a = lo ? a : zero;
+    // lanes 32-63 supply K slots 8-15, whose B rows are zero: their A entries (the same
+    // candidate's, finite) add exactly 0 and need no masking
     // next word's rows, requested once this word's are used up (the empty asm pins the A
     // operand before the request, so this word does not wait for the next one's load)
     asm volatile("" ::"v"(a));
