@@ -3,6 +3,7 @@
 #include "particle.h"
 
 #include <algorithm>
+#include <functional>
 #include <iostream>
 #include <random>
 
@@ -35,22 +36,40 @@ void ParticleSystem::add_particles(const std::vector<Particle>& new_particles) {
     for (const auto& p : new_particles) add_particle(p);
 }
 
-// src/particle.cpp:41-66
+// Removal semantics of src/particle.cpp:41-66.  remove_particle renumbers the particles from
+// `index` on to their new positions; remove_particles applies its indices largest first, each
+// to the container as it stands (one at or past the current end is skipped, a repeated one
+// removes the particle that has moved into its place), then renumbers every particle.
 void ParticleSystem::remove_particle(size_t index) {
-    if (index < particles_.size()) {
-        particles_.erase(particles_.begin() + static_cast<std::ptrdiff_t>(index));
-        for (size_t i = index; i < particles_.size(); ++i) particles_[i].id = static_cast<int>(i);
-    }
+    if (index >= particles_.size()) return;
+    std::move(particles_.begin() + static_cast<std::ptrdiff_t>(index) + 1, particles_.end(),
+              particles_.begin() + static_cast<std::ptrdiff_t>(index));
+    particles_.pop_back();
+    for (size_t i = index; i < particles_.size(); ++i) particles_[i].id = static_cast<int>(i);
 }
 
 void ParticleSystem::remove_particles(const std::vector<size_t>& indices) {
-    std::vector<size_t> sorted = indices;
-    std::sort(sorted.rbegin(), sorted.rend());
-    for (size_t index : sorted) {
-        if (index < particles_.size())
-            particles_.erase(particles_.begin() + static_cast<std::ptrdiff_t>(index));
+    std::vector<size_t> todo(indices);
+    std::sort(todo.begin(), todo.end(), std::greater<size_t>());
+    // Only positions at or after an index have moved when it is applied (the larger ones went
+    // first), so position i then holds the first particle at original index >= i not yet gone.
+    std::vector<char> gone(particles_.size(), 0);
+    size_t live = particles_.size();
+    for (size_t i : todo) {
+        if (i >= live) continue;
+        size_t k = i;
+        while (gone[k]) ++k;
+        gone[k] = 1;
+        --live;
     }
-    for (size_t i = 0; i < particles_.size(); ++i) particles_[i].id = static_cast<int>(i);
+    size_t w = 0;
+    for (size_t k = 0; k < particles_.size(); ++k) {
+        if (gone[k]) continue;
+        if (w != k) particles_[w] = std::move(particles_[k]);
+        particles_[w].id = static_cast<int>(w);
+        ++w;
+    }
+    particles_.resize(w);
 }
 
 std::vector<vec3> ParticleSystem::get_positions() const {
@@ -81,16 +100,21 @@ std::vector<float> ParticleSystem::get_pressures() const {
     return out;
 }
 
-void ParticleSystem::set_mass(float mass) {
-    for (auto& p : particles_) p.mass = mass;
+namespace {
+// one field of every particle set to `v` (the set_* calls of src/particle.cpp:104-120)
+void assign_field(std::vector<Particle>& ps, float Particle::*field, float v) {
+    std::for_each(ps.begin(), ps.end(), [field, v](Particle& p) { p.*field = v; });
 }
+}  // namespace
+
+void ParticleSystem::set_mass(float mass) { assign_field(particles_, &Particle::mass, mass); }
 
 void ParticleSystem::set_viscosity(float viscosity) {
-    for (auto& p : particles_) p.viscosity = viscosity;
+    assign_field(particles_, &Particle::viscosity, viscosity);
 }
 
 void ParticleSystem::set_temperature(float temperature) {
-    for (auto& p : particles_) p.temperature = temperature;
+    assign_field(particles_, &Particle::temperature, temperature);
 }
 
 // Host-side wall clamp of the mirror (src/particle.cpp:122-153).  The engine's own clamp
@@ -113,15 +137,18 @@ void ParticleSystem::apply_boundary_conditions(float xmin, float xmax, float ymi
     }
 }
 
-// src/particle.cpp:156-164
+// A point uniform in the box [xmin, xmax) x [ymin, ymax) x [zmin, zmax) (src/particle.cpp:156-165):
+// one generator per thread, seeded once from the system's entropy source; coordinates drawn
+// x, then y, then z.
 vec3 generate_random_position(float xmin, float xmax, float ymin, float ymax, float zmin,
                               float zmax) {
-    static std::random_device rd;
-    static std::mt19937 gen(rd());
-    std::uniform_real_distribution<float> dx(xmin, xmax), dy(ymin, ymax), dz(zmin, zmax);
-    float x = dx(gen);
-    float y = dy(gen);
-    float z = dz(gen);
+    thread_local std::mt19937 engine{std::random_device{}()};
+    auto draw = [](float lo, float hi) {
+        return std::uniform_real_distribution<float>(lo, hi)(engine);
+    };
+    const float x = draw(xmin, xmax);
+    const float y = draw(ymin, ymax);
+    const float z = draw(zmin, zmax);
     return vec3(x, y, z);
 }
 

@@ -1,7 +1,7 @@
 """Host-side code under AddressSanitizer and UndefinedBehaviorSanitizer: the CPU oracle (the
 step in both formulations, neighbour lists against brute force, non-finite and out-of-range
-coordinates, the FP64 evaluation, the surrogate's forward/gradient/Adam) and the host kernel
-classes.  Built with gcc/g++ from tests/native/Makefile; no GPU involved."""
+coordinates, the FP64 evaluation, the surrogate's forward/gradient/Adam), the host kernel
+classes and the host particle container (removal semantics against the reference's).  Built with gcc/g++ from tests/native/Makefile; no GPU involved."""
 import os
 import shutil
 import subprocess
@@ -21,7 +21,7 @@ def binaries():
     return os.path.join(NATIVE, "_build")
 
 
-@pytest.mark.parametrize("name", ["oracle_sanitize", "kernels_sanitize"])
+@pytest.mark.parametrize("name", ["oracle_sanitize", "kernels_sanitize", "particles_check"])
 def test_sanitized_run(binaries, name):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
                UBSAN_OPTIONS="print_stacktrace=1", OMP_NUM_THREADS="4")
