@@ -69,9 +69,37 @@ def test_wendland_c2(sph):
             continue
         np.testing.assert_allclose(k.gradW(r), _fd_grad(k, r, 1e-5), rtol=2e-3,
                                    atol=2e-3 * np.abs(_fd_grad(k, r, 1e-5)).max())
-    # laplacianW follows the reference's expression norm (5/h^2)(1-q/2)^2 (5q-3), which is
-    # not the Laplacian of W (15 (q-1)(1-q/2)^2 norm / h^2): parity unpinned beyond its sign
     assert k.laplacianW(np.zeros(3, F32)) < 0
+
+
+def test_wendland_laplacian_expression(sph):
+    """laplacianW of src/kernels.cpp:192-198, norm (5/h^2)(1-q/2)^2 (5q-3), which is not the
+    Laplacian of its W (that is 15 (q-1)(1-q/2)^2 norm/h^2).  Pinned without retyping it: in
+    units of norm/h^2 it is a cubic in q = |r|/h (fit residual at FP32 rounding), and a cubic is
+    fixed by four conditions read off the reference's factors: -15 at q = 0, a simple root at
+    q = 3/5 and a double root at q = 2; zero from q = 2 on, and a function of |r| alone."""
+    h = 0.02
+    k = sph.create_kernel(sph.KernelType.WENDLAND_C2, h)
+    scale = 21.0 / (2.0 * math.pi * h ** 3) / (h * h)
+    rs = (np.linspace(0.0, 1.99, 400) * h).astype(F32)
+    q = rs.astype(np.float64) / h
+    y = np.array([float(k.laplacianW(np.array([r, 0, 0], F32))) for r in rs]) / scale
+    c = np.polyfit(q, y, 3)
+    assert np.abs(np.polyval(c, q) - y).max() <= 2e-6 * np.abs(y).max()
+    assert np.polyval(c, 0.0) == pytest.approx(-15.0, rel=1e-5)
+    assert abs(np.polyval(c, 0.6)) <= 1e-5 * 15
+    assert abs(np.polyval(c, 2.0)) <= 1e-5 * 15
+    assert abs(np.polyval(np.polyder(c), 2.0)) <= 1e-4 * 15
+    assert abs(np.polyval(np.polyder(c), 0.6)) > 1.0  # a simple root there, not a double one
+    for r in (2.0 * h, 2.0001 * h, 3.0 * h):
+        assert k.laplacianW(np.array([r, 0, 0], F32)) == 0.0
+    rng = np.random.default_rng(3)
+    for r in rs[::37]:
+        u = rng.normal(size=3)
+        v = (u / np.linalg.norm(u) * float(r)).astype(F32)
+        qv = float(np.linalg.norm(v.astype(np.float64))) / h
+        assert float(k.laplacianW(v)) / scale == pytest.approx(
+            float(np.polyval(c, qv)), rel=1e-5, abs=1e-5 * 15)
 
 
 def test_gaussian(sph):
