@@ -171,7 +171,7 @@ __device__ __forceinline__ float density_bits_deep(BitWalk<kSeg>& bw, uint32_t s
 #pragma unroll
         for (int k = 0; k <= kAhead; ++k) {
             q[(k + kAhead) % (kAhead + 1)] = row_at(spm, bw.template next<4>(lmask, lwj0, lane));
-            acc = density_pair<kCut>(pi, q[k], c, hh, acc);
+            acc = pinned(density_pair<kCut>(pi, q[k], c, hh, acc));
         }
     }
 #pragma unroll
@@ -511,6 +511,11 @@ struct ForceSums {
         fv.y += cv * (b.y - vi.y);
         fv.z += cv * (b.z - vi.z);
     }
+    // the sums (and the count) formed here, before whatever follows (see pinned)
+    __device__ __forceinline__ void pin(uint32_t& cnt) {
+        asm volatile("" : "+v"(fp.x), "+v"(fp.y), "+v"(fp.z), "+v"(fv.x), "+v"(fv.y),
+                     "+v"(fv.z), "+v"(cnt));
+    }
 };
 
 // The wavefront runs its largest lane count; rows are requested two neighbours ahead, in
@@ -572,6 +577,7 @@ __device__ __forceinline__ void force_bits_deep(BitWalk<kSeg>& bw, uint32_t step
             a[(k + kAhead) % (kAhead + 1)] = row_at(rows, o).a;
             b[(k + kAhead) % (kAhead + 1)] = row_at(rows, o).b;
             f.template add<kCut>(pi, vi, p_quarter, a[k], b[k], c, cnt);
+            f.pin(cnt);
         }
     }
 #pragma unroll
