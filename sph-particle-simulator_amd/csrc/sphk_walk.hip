@@ -180,8 +180,13 @@ __device__ __forceinline__ float density_bits_deep(BitWalk<kSeg>& bw, uint32_t s
     return acc;
 }
 
-// Rows in flight per wavefront in the split shapes' walks.
+// Rows in flight per wavefront in the split shapes' walks, and per lane in the ping-pong force
+// walk.  Measured on the 1M dam break (pp16_waves4): 2 rows 0.578 ms, 3 rows 0.574, 4 rows
+// 0.566, 5 rows 0.561, 6 rows 0.560 (158 VGPRs: three wavefronts per SIMD where the LDS allows
+// four), 8 rows 0.624 (two wavefronts per SIMD): the force walk is bound by its gathers'
+// latency more than by wavefronts in flight.
 constexpr int kDeepAhead = 5;
+constexpr int kPPAhead = 6;
 
 // The ping-pong shapes' walk, which goes on across calls: on entry and on return the rows of
 // the lane's next two pairs are in flight in q0 and q1, and a call takes exactly `steps` bits
@@ -585,39 +590,39 @@ __device__ __forceinline__ void force_bits_deep(BitWalk<kSeg>& bw, uint32_t step
         if (t + k < steps) f.template add<kCut>(pi, vi, p_quarter, a[k], b[k], c, cnt);
 }
 
-// force_bits for the ping-pong shapes: the walk goes on across calls with the rows of the next
-// two pairs in flight (a0/b0, a1/b1) on entry and on return; a call takes exactly `steps`
-// bits and adds exactly `steps` pairs.
-template <int kW, bool kCut>
-__device__ __forceinline__ void force_run(BitWalk<kW>& bw, float4& a0, float4& b0, float4& a1,
-                                          float4& b1, uint32_t steps,
-                                          const uint32_t* __restrict__ lmask,
-                                          const uint32_t* __restrict__ lwj0, uint32_t lane,
-                                          const ForceRow* __restrict__ rows, const float4& pi,
-                                          const float3& vi, float p_quarter, const StepConsts& c,
-                                          ForceSums& f, uint32_t& cnt) {
+// force_bits for the ping-pong shapes: the walk goes on across calls with the rows of the
+// lane's next kAhead pairs in flight (slots 0 .. kAhead - 1 on entry and on return, slot
+// kAhead free); a call takes exactly `steps` bits and adds exactly `steps` pairs.  Each step
+// requests into the free slot and sums the oldest, so over a block of kAhead + 1 steps the
+// slots come back to their order and no row set is copied between blocks.
+template <int kW, bool kCut, int kAhead>
+__device__ __forceinline__ void force_run_n(BitWalk<kW>& bw, float4 (&a)[kAhead + 1],
+                                            float4 (&b)[kAhead + 1], uint32_t steps,
+                                            const uint32_t* __restrict__ lmask,
+                                            const uint32_t* __restrict__ lwj0, uint32_t lane,
+                                            const ForceRow* __restrict__ rows, const float4& pi,
+                                            const float3& vi, float p_quarter,
+                                            const StepConsts& c, ForceSums& f, uint32_t& cnt) {
     uint32_t t = 0;
-    for (; t + 3 <= steps; t += 3) {
-        uint32_t o = bw.template next<5>(lmask, lwj0, lane);
-        const float4 a2 = row_at(rows, o).a, b2 = row_at(rows, o).b;
-        f.template add<kCut>(pi, vi, p_quarter, a0, b0, c, cnt);
-        o = bw.template next<5>(lmask, lwj0, lane);
-        a0 = row_at(rows, o).a;
-        b0 = row_at(rows, o).b;
-        f.template add<kCut>(pi, vi, p_quarter, a1, b1, c, cnt);
-        o = bw.template next<5>(lmask, lwj0, lane);
-        a1 = row_at(rows, o).a;
-        b1 = row_at(rows, o).b;
-        f.template add<kCut>(pi, vi, p_quarter, a2, b2, c, cnt);
+    for (; t + kAhead + 1 <= steps; t += kAhead + 1) {
+#pragma unroll
+        for (int k = 0; k <= kAhead; ++k) {
+            const uint32_t o = bw.template next<5>(lmask, lwj0, lane);
+            a[(k + kAhead) % (kAhead + 1)] = row_at(rows, o).a;
+            b[(k + kAhead) % (kAhead + 1)] = row_at(rows, o).b;
+            f.template add<kCut>(pi, vi, p_quarter, a[k], b[k], c, cnt);
+        }
     }
     for (; t < steps; ++t) {
         const uint32_t o = bw.template next<5>(lmask, lwj0, lane);
-        const float4 a2 = row_at(rows, o).a, b2 = row_at(rows, o).b;
-        f.template add<kCut>(pi, vi, p_quarter, a0, b0, c, cnt);
-        a0 = a1;
-        b0 = b1;
-        a1 = a2;
-        b1 = b2;
+        a[kAhead] = row_at(rows, o).a;
+        b[kAhead] = row_at(rows, o).b;
+        f.template add<kCut>(pi, vi, p_quarter, a[0], b[0], c, cnt);
+#pragma unroll
+        for (int k = 0; k < kAhead; ++k) {
+            a[k] = a[k + 1];
+            b[k] = b[k + 1];
+        }
     }
 }
 
@@ -750,17 +755,17 @@ void k_force_walk(
         uint32_t* const lj_new = lwj0 + kSeg;
         PingPong<kSeg> pp;
         pp.init(c.pad_slot);
-        float4 a0, b0, a1, b1;  // rows of the next two pairs, requested (within a phase)
+        float4 a[kPPAhead + 1], b[kPPAhead + 1];  // rows requested ahead (within a phase)
         bool more = true, pend = false;  // pend: the newer buffer holds the next part's segment
         int pend_part = 0, pend_nw = 0, walk_part = -1;
-        auto request_two = [&]() {
-            uint32_t o = pp.bw.template next<5>(lmask, lwj0, lane);
-            a0 = row_at(rows, o).a;
-            b0 = row_at(rows, o).b;
-            o = pp.bw.template next<5>(lmask, lwj0, lane);
-            a1 = row_at(rows, o).a;
-            b1 = row_at(rows, o).b;
-            pp.walked(2);
+        auto request_ahead = [&]() {
+#pragma unroll
+            for (int k = 0; k < kPPAhead; ++k) {
+                const uint32_t o = pp.bw.template next<5>(lmask, lwj0, lane);
+                a[k] = row_at(rows, o).a;
+                b[k] = row_at(rows, o).b;
+            }
+            pp.walked(kPPAhead);
         };
         for (;;) {
             if (pp.nw_new == 0 && !pend && more) {
@@ -816,13 +821,14 @@ void k_force_walk(
             const bool overlap = pp.nw_new != 0;
             if (!overlap && walk_part < 0) break;  // no segment at all
             if ((overlap ? pp.older_steps() : pp.all_steps()) != 0) {
-                request_two();
+                request_ahead();
                 const uint32_t steps = overlap ? pp.older_steps() : pp.all_steps();
-                force_run<kLds, kCut>(pp.bw, a0, b0, a1, b1, steps, lmask, lwj0, lane, rows, pi,
-                                      vi, p_quarter, c, f, cnt);
+                force_run_n<kLds, kCut, kPPAhead>(pp.bw, a, b, steps, lmask, lwj0, lane, rows,
+                                                  pi, vi, p_quarter, c, f, cnt);
                 pp.walked(steps);
-                f.template add<kCut>(pi, vi, p_quarter, a0, b0, c, cnt);
-                f.template add<kCut>(pi, vi, p_quarter, a1, b1, c, cnt);
+#pragma unroll
+                for (int k = 0; k < kPPAhead; ++k)
+                    f.template add<kCut>(pi, vi, p_quarter, a[k], b[k], c, cnt);
             }
             if (overlap) {
                 pp.shift_down(lmask, lwj0, lane);
