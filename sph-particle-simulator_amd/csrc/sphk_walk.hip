@@ -116,8 +116,9 @@ __device__ __forceinline__ float density_pair(const float4& pi, const float4& q,
 // the next step's refill branch, so the row it used is dead when the row after it is requested
 // and the two share registers; left to the compiler, the sums sink past the branches and the
 // loop copies a row set at its back edge.  Measured on the 1M dam break: density walk 0.429
-// -> 0.423 ms.  The force walk keeps its sums unpinned: its steps wait on row gathers, and
-// pinned sums (which hold the next request back until they are formed) took it 0.581 -> 0.594.
+// -> 0.423 ms; at 10,000 particles the split shapes' walks (both pinned) 1-2 % faster.  The
+// ping-pong force walk keeps its sums unpinned: its steps wait on row gathers, and pinned sums
+// (which hold the next request back until they are formed) took it 0.581 -> 0.594 ms.
 __device__ __forceinline__ float pinned(float x) {
     asm volatile("" : "+v"(x));
     return x;
